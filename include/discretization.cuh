@@ -1,0 +1,63 @@
+/**
+ \file  discretization.cuh
+ \brief Host entry points of the step (drop-in for src/discretization.cuh:34-118).
+
+ device_evolution launches one fused kernel for the whole step (boundaries,
+ fictitious compositions, Laplacians, Cahn-Hilliard, Allen-Cahn); the four
+ calls the driver makes before it are kept for the call sequence of
+ src/main.cpp:289-303 and do no device work.  All work goes to dev->str_A in
+ order, so the calls are ordered without host synchronisation; results are
+ complete when read_out_result / device_dataviz return.
+*/
+#ifndef _CUDA_KERNELS_H_
+#define _CUDA_KERNELS_H_
+
+#include "mesh.h"
+
+/** Tiles along an axis for a block of tile_loc with a mask_size halo (src/discretization.cu:45-48). */
+float nTiles(int domain_size, int tile_loc, int mask_size);
+
+void device_fictitious(struct CudaData* dev, const int nx, const int ny, const int nm,
+                       const int bx, const int by);
+
+void device_boundaries(struct CudaData* dev, const int nx, const int ny, const int nm,
+                       const int bx, const int by);
+
+void fictitious_boundaries(struct CudaData* dev, const int nx, const int ny, const int nm,
+                           const int bx, const int by);
+
+void device_laplacian_boundaries(struct CudaData* dev, const int nx, const int ny, const int nm,
+                                 const int bx, const int by);
+
+void device_laplacian(struct CudaData* dev, const int nx, const int ny, const int nm,
+                      const int bx, const int by, const fp_t dx, const fp_t dy);
+
+/** One explicit Euler step *_old -> *_new (src/discretization.cu:982-1020). */
+void device_evolution(struct CudaData* dev, const int nx, const int ny, const int nm,
+                      const int bx, const int by, const fp_t alpha, const fp_t dt);
+
+/** Key the nucleation random stream (src/discretization.cu:1033-1044). */
+void device_init_prng(struct CudaData* dev, const int nx, const int ny, const int nm,
+                      const int bx, const int by);
+
+/** Nucleation pass on *_new (src/discretization.cu:1179-1209). */
+void device_nucleation(struct CudaData* dev, const int nx, const int ny, const int nm,
+                       const int bx, const int by,
+                       const fp_t sigma_del, const fp_t sigma_lav,
+                       const fp_t unit_a, const fp_t ifce_width,
+                       const fp_t dx, const fp_t dy, const fp_t dz, const fp_t dt);
+
+/** x_Ni and phi of the latest fields, copied to the host with the fields
+ *  (src/discretization.cu:1231-1263). */
+void device_dataviz(struct CudaData* dev, struct HostData* host, const int nx, const int ny,
+                    const int nm, const int bx, const int by);
+
+/* ---- additions with no counterpart in the reference ---- */
+
+/** Boundary mode of subsequent steps: 0 no-flux (default, the reference's), 1 periodic. */
+void device_set_boundary(struct CudaData* dev, int bc);
+
+/** Seed of the nucleation stream (default 42); resets the stream position. */
+void device_seed_prng(struct CudaData* dev, uint64_t seed);
+
+#endif /* _CUDA_KERNELS_H_ */

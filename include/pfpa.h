@@ -1,0 +1,156 @@
+/*
+ * pfpa.h -- C-ABI of the MI355X-native KKS phase-field step (libpfpa.so).
+ *
+ * Plain pointers and sizes only.  Field pointers are DEVICE pointers to
+ * (nx x ny) row-major arrays with the reference's one-cell ghost ring
+ * (nx = Nx + 2, idx = nx*y + x), as src/type.h:58-107 lays them out.
+ * `stream` is a hipStream_t (NULL = the default stream).  Every call returns
+ * PF_OK or an error code; pf_last_error() describes the last failure of the
+ * calling thread.  Calls are asynchronous on `stream` unless noted.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the reference repository root).
+ */
+#ifndef PFPA_H
+#define PFPA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PF_OK 0
+#define PF_ERR_ARG 1         /* invalid argument (sizes, null pointers, mode) */
+#define PF_ERR_HIP 2         /* HIP runtime error */
+#define PF_ERR_OVERFLOW 3    /* nucleation event list overflowed */
+#define PF_ERR_NO_DEVICE 4   /* no GPU visible */
+
+#define PF_BC_NOFLUX 0       /* ghost := adjacent interior (src/discretization.cu:360-394) */
+#define PF_BC_PERIODIC 1
+
+/* Kinetic and discretisation parameters of a step: the values init_cuda
+ * uploads to __constant__ memory (src/data.cu:70-80) plus the dt and alpha
+ * that device_evolution takes (src/discretization.cuh:89-93). */
+typedef struct pf_params {
+	double dt;
+	double DCr[2];    /* d_DCr: row of the constant diffusivity matrix */
+	double DNb[2];    /* d_DNb */
+	double kappa[2];  /* d_Kapp: gradient coefficients (delta, Laves) */
+	double omega[2];  /* d_Omeg: well heights */
+	double Lmob[2];   /* d_Lmob: Allen-Cahn mobilities */
+	double alpha;     /* three-phase coefficient Omega */
+	double mask[9];   /* d_mask: 3x3 Laplacian mask, row-major (src/mesh.cpp:187-196) */
+} pf_params;
+
+/* Arguments of device_nucleation (src/discretization.cuh:105-111), plus the
+ * counter-based random stream that replaces the per-cell cuRAND state. */
+typedef struct pf_nuc_params {
+	double sigma_del, sigma_lav;  /* interfacial energies (J/m^2) */
+	double unit_a;                /* lattice constant (m) */
+	double ifce_width;            /* accepted, unused (as in the reference) */
+	double dx, dy, dz;            /* mesh spacing; the pass uses dx*nx, dy*ny (src/discretization.cu:1206) */
+	double dt;                    /* nuc_interval * dt (src/main.cpp:314) */
+	uint64_t seed;                /* Philox4x32-10 key */
+	uint64_t call;                /* stream position: index of this nucleation pass */
+	uint32_t instance;            /* simulation id folded into the counter */
+	uint32_t reserved;
+} pf_nuc_params;
+
+typedef struct pf_ctx pf_ctx;
+
+/* Context: device scratch (nucleation event list, reduction partials) and
+ * tile settings.  Replaces the allocation half of init_cuda / free_cuda
+ * (src/data.cu:10-154). */
+int pf_ctx_create(pf_ctx** ctx);
+void pf_ctx_destroy(pf_ctx* ctx);
+const char* pf_last_error(void);
+int pf_device_count(void);
+/* rows per block of the marching step kernel (0 = heuristic), XCD-aware block order on/off */
+int pf_ctx_set_tile(pf_ctx* ctx, int rows, int xcd_remap);
+
+/* One fused explicit-Euler step, old -> new.  Replaces the per-step sequence
+ * device_boundaries, device_fictitious, fictitious_boundaries,
+ * device_laplacian, device_laplacian_boundaries, device_evolution
+ * (src/main.cpp:289-305; src/discretization.cu:128-173,396-548,780-1020).
+ * nuc != NULL fuses the nucleation pass on the new fields (src/main.cpp:307-316).
+ * Interior cells and the ghost ring of the outputs are written. */
+int pf_step_f64(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm,
+                const double* conc_Cr_old, const double* conc_Nb_old,
+                const double* phi_del_old, const double* phi_lav_old,
+                double* conc_Cr_new, double* conc_Nb_new,
+                double* phi_del_new, double* phi_lav_new,
+                const pf_nuc_params* nuc, void* stream);
+
+/* Precision form: fields stored in FP32, arithmetic (KKS solve included) in FP64. */
+int pf_step_f32(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm,
+                const float* conc_Cr_old, const float* conc_Nb_old,
+                const float* phi_del_old, const float* phi_lav_old,
+                float* conc_Cr_new, float* conc_Nb_new,
+                float* phi_del_new, float* phi_lav_new,
+                const pf_nuc_params* nuc, void* stream);
+
+/* Standalone nucleation pass in place on the given fields.  Replaces
+ * device_nucleation (src/discretization.cu:1179-1209). */
+int pf_nucleation_f64(pf_ctx* ctx, int nx, int ny, double* conc_Cr, double* conc_Nb,
+                      double* phi_del, double* phi_lav, const pf_nuc_params* nuc, void* stream);
+int pf_nucleation_f32(pf_ctx* ctx, int nx, int ny, float* conc_Cr, float* conc_Nb,
+                      float* phi_del, float* phi_lav, const pf_nuc_params* nuc, void* stream);
+
+/* Synchronous: events recorded since the context was created that needed a
+ * deferred (radius >= 1 cell) embed, and whether the event list overflowed. */
+int pf_nucleation_status(pf_ctx* ctx, uint64_t* wide_events, int* overflow);
+
+/* Means over interior cells of {x_Cr, x_Nb, gamma, delta, Laves} into the
+ * device array out5[5]: deterministic compensated FP64 sums.  Replaces
+ * summarize_fields (src/alloy625.cpp:811-863). */
+int pf_summarize_f64(pf_ctx* ctx, int nx, int ny, const double* conc_Cr, const double* conc_Nb,
+                     const double* phi_del, const double* phi_lav, double* out5, void* stream);
+int pf_summarize_f32(pf_ctx* ctx, int nx, int ny, const float* conc_Cr, const float* conc_Nb,
+                     const float* phi_del, const float* phi_lav, double* out5, void* stream);
+
+/* x_Ni = 1 - x_Cr - x_Nb and phi = p(phi_del) + p(phi_lav) over every cell.
+ * Replaces dataviz_kernel (src/discretization.cu:1211-1229). */
+int pf_dataviz_f64(int nx, int ny, const double* conc_Cr, const double* conc_Nb,
+                   const double* phi_del, const double* phi_lav, double* conc_Ni,
+                   double* phi, void* stream);
+
+/* Pointwise device thermodynamics over n inputs (device arrays), the
+ * __device__ API of src/parabola625.cuh:13-95 and src/nucleation.cuh:8-20:
+ *   what 0: in x[n]                         -> out {p, pPrime}[n][2]
+ *   what 1: in {xCr,xNb,pDel,pGam,pLav}[n]  -> out fict {gam,del,lav}x{Cr,Nb}[n][6]
+ *   what 2: in {xCr,xNb}[n]                 -> out {g x3, dg x6, D_CrCr,D_CrNb,D_NbCr,D_NbNb}[n][13]
+ *   what 3: in nucleation_probability_sphere arguments[n][10] -> out {Rstar, P}[n][2] */
+int pf_thermo_eval(int what, int n, const double* in, double* out, void* stream);
+
+/* Philox4x32-10 over n {ctr0, ctr1, ctr2, ctr3, key0, key1} tuples -> n x 4
+ * words (device arrays).  Replaces the cuRAND XORWOW state of
+ * init_prng_kernel (src/discretization.cu:1022-1044). */
+int pf_philox_eval(int n, const uint32_t* in, uint32_t* out, void* stream);
+
+/* Heuristic rows-per-block the step kernel uses for an nx x ny grid. */
+int pf_default_rows(int nx, int ny);
+
+/* Host-side evaluation of the same pointwise thermodynamics (no GPU needed).
+ * Replace the host functions of thermo/parabola625.h the driver calls:
+ * p (src/main.cpp:340), fict_gam_Cr/Nb (src/main.cpp:346-347) and
+ * D_CrCr/D_CrNb/D_NbCr/D_NbNb (src/alloy625.cpp:91-95). */
+double pf_host_p(double x);
+double pf_host_pPrime(double x);
+double pf_host_g(int phase, double xCr, double xNb);           /* phase 0 gamma, 1 delta, 2 Laves */
+double pf_host_D(int i, int j, double xCr, double xNb);        /* i, j: 0 Cr, 1 Nb */
+void pf_host_fict(double xCr, double xNb, double pDel, double pGam, double pLav, double* out6);
+double pf_host_xe(int phase, int comp);
+void pf_host_nucleation_probability(double xCr, double xNb, double dG, double DCrCr,
+                                    double DNbNb, double sigma, double Vatom, double n_gam,
+                                    double dV, double dt, double* Rstar, double* P);
+/* Largest eigenvalue over the three phases of D * H_gamma^-1 * H_a: the
+ * effective diffusivity that bounds the explicit step inside each phase. */
+double pf_host_max_effective_diffusivity(const double* DCr, const double* DNb);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFPA_H */

@@ -1,0 +1,279 @@
+// pf_capi.hip -- extern "C" entry points of libpfpa.so (declared in include/pfpa.h).
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "pfpa.h"
+#include "pf_internal.h"
+#include "pf_thermo.h"
+
+struct pf_ctx {
+  pf::Scratch scratch;
+  pf::TileConfig tile;
+  int device;
+};
+
+namespace {
+
+int fail(int code, const std::string& msg) {
+  pf::set_error(msg);
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* where) {
+  return fail(PF_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+int check_grid(int nx, int ny) {
+  if (nx < 3 || ny < 3) return fail(PF_ERR_ARG, "grid must have at least one interior cell (nx, ny >= 3)");
+  if ((long long)nx * (long long)ny >= (1LL << 31))
+    return fail(PF_ERR_ARG, "grid too large for 32-bit cell indices");
+  return PF_OK;
+}
+
+int check_ptrs(const void* a, const void* b, const void* c, const void* d) {
+  if (!a || !b || !c || !d) return fail(PF_ERR_ARG, "null field pointer");
+  return PF_OK;
+}
+
+template <typename T>
+int step_t(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const T* iCr,
+           const T* iNb, const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd, T* opl,
+           const pf_nuc_params* nuc, void* stream) {
+  if (!ctx || !prm) return fail(PF_ERR_ARG, "null context or parameters");
+  if (int r = check_grid(nx, ny)) return r;
+  if (int r = check_ptrs(iCr, iNb, ipd, ipl)) return r;
+  if (int r = check_ptrs(oCr, oNb, opd, opl)) return r;
+  if (bc != PF_BC_NOFLUX && bc != PF_BC_PERIODIC) return fail(PF_ERR_ARG, "unknown boundary mode");
+  if ((const void*)iCr == (const void*)oCr || (const void*)iNb == (const void*)oNb ||
+      (const void*)ipd == (const void*)opd || (const void*)ipl == (const void*)opl)
+    return fail(PF_ERR_ARG, "step cannot run in place: old and new buffers must differ");
+  const pf::StepParams P = pf::make_step(prm->dt, prm->DCr, prm->DNb, prm->kappa, prm->omega,
+                                         prm->Lmob, prm->alpha, prm->mask);
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e;
+  if (nuc) {
+    const pf::NucParams N =
+        pf::make_nuc(nuc->sigma_del, nuc->sigma_lav, nuc->unit_a, nuc->dx, nuc->dy, nuc->dz,
+                     nuc->dt, nuc->seed, nuc->call, nuc->instance, nx, ny);
+    if (N.R_scan < 1.0) {
+      // fused: decisions on each cell's new values, centre embeds in place,
+      // wider embeds deferred to the event pass
+      e = pf::launch_step<T>(nx, ny, bc, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, &N,
+                             ctx->scratch, ctx->tile, st);
+      if (e != hipSuccess) return hip_fail(e, "step kernel");
+      e = pf::launch_apply_events<T>(nx, ny, oCr, oNb, opd, opl, ctx->scratch, st);
+      if (e != hipSuccess) return hip_fail(e, "nucleation embed kernel");
+      return PF_OK;
+    }
+    e = pf::launch_step<T>(nx, ny, bc, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, nullptr,
+                           ctx->scratch, ctx->tile, st);
+    if (e != hipSuccess) return hip_fail(e, "step kernel");
+    e = pf::launch_nucleation<T>(nx, ny, oCr, oNb, opd, opl, N, ctx->scratch, st);
+    if (e != hipSuccess) return hip_fail(e, "nucleation kernel");
+    return PF_OK;
+  }
+  e = pf::launch_step<T>(nx, ny, bc, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, nullptr,
+                         ctx->scratch, ctx->tile, st);
+  if (e != hipSuccess) return hip_fail(e, "step kernel");
+  return PF_OK;
+}
+
+template <typename T>
+int nucleation_t(pf_ctx* ctx, int nx, int ny, T* xCr, T* xNb, T* pd, T* pl,
+                 const pf_nuc_params* nuc, void* stream) {
+  if (!ctx || !nuc) return fail(PF_ERR_ARG, "null context or parameters");
+  if (int r = check_grid(nx, ny)) return r;
+  if (int r = check_ptrs(xCr, xNb, pd, pl)) return r;
+  const pf::NucParams N =
+      pf::make_nuc(nuc->sigma_del, nuc->sigma_lav, nuc->unit_a, nuc->dx, nuc->dy, nuc->dz,
+                   nuc->dt, nuc->seed, nuc->call, nuc->instance, nx, ny);
+  hipError_t e =
+      pf::launch_nucleation<T>(nx, ny, xCr, xNb, pd, pl, N, ctx->scratch, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "nucleation kernel");
+  return PF_OK;
+}
+
+template <typename T>
+int summarize_t(pf_ctx* ctx, int nx, int ny, const T* xCr, const T* xNb, const T* pd,
+                const T* pl, double* out5, void* stream) {
+  if (!ctx || !out5) return fail(PF_ERR_ARG, "null context or output");
+  if (int r = check_grid(nx, ny)) return r;
+  if (int r = check_ptrs(xCr, xNb, pd, pl)) return r;
+  hipError_t e = pf::launch_summarize<T>(nx, ny, xCr, xNb, pd, pl, out5, ctx->scratch,
+                                         (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "summarize kernels");
+  return PF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pf_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int pf_ctx_create(pf_ctx** out) {
+  if (!out) return fail(PF_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (pf_device_count() < 1) return fail(PF_ERR_NO_DEVICE, "no HIP device visible");
+  pf_ctx* c = new pf_ctx();
+  c->scratch = pf::Scratch{};
+  if (hipGetDevice(&c->device) != hipSuccess) c->device = 0;
+  hipError_t e = pf::scratch_alloc(c->scratch);
+  if (e != hipSuccess) {
+    pf::scratch_free(c->scratch);
+    delete c;
+    return hip_fail(e, "scratch allocation");
+  }
+  *out = c;
+  return PF_OK;
+}
+
+void pf_ctx_destroy(pf_ctx* ctx) {
+  if (!ctx) return;
+  pf::scratch_free(ctx->scratch);
+  delete ctx;
+}
+
+const char* pf_last_error(void) { return pf::last_error(); }
+
+int pf_ctx_set_tile(pf_ctx* ctx, int rows, int xcd_remap) {
+  if (!ctx) return fail(PF_ERR_ARG, "null context");
+  if (rows < 0 || rows > 1 << 20) return fail(PF_ERR_ARG, "rows out of range");
+  ctx->tile.rows = rows;
+  ctx->tile.remap = xcd_remap ? 1 : 0;
+  return PF_OK;
+}
+
+int pf_default_rows(int nx, int ny) { return pf::default_rows(nx, ny); }
+
+int pf_step_f64(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const double* a,
+                const double* b, const double* c, const double* d, double* oa, double* ob,
+                double* oc, double* od, const pf_nuc_params* nuc, void* stream) {
+  return step_t<double>(ctx, nx, ny, bc, prm, a, b, c, d, oa, ob, oc, od, nuc, stream);
+}
+
+int pf_step_f32(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const float* a,
+                const float* b, const float* c, const float* d, float* oa, float* ob, float* oc,
+                float* od, const pf_nuc_params* nuc, void* stream) {
+  return step_t<float>(ctx, nx, ny, bc, prm, a, b, c, d, oa, ob, oc, od, nuc, stream);
+}
+
+int pf_nucleation_f64(pf_ctx* ctx, int nx, int ny, double* a, double* b, double* c, double* d,
+                      const pf_nuc_params* nuc, void* stream) {
+  return nucleation_t<double>(ctx, nx, ny, a, b, c, d, nuc, stream);
+}
+
+int pf_nucleation_f32(pf_ctx* ctx, int nx, int ny, float* a, float* b, float* c, float* d,
+                      const pf_nuc_params* nuc, void* stream) {
+  return nucleation_t<float>(ctx, nx, ny, a, b, c, d, nuc, stream);
+}
+
+int pf_nucleation_status(pf_ctx* ctx, uint64_t* wide_events, int* overflow) {
+  if (!ctx) return fail(PF_ERR_ARG, "null context");
+  uint32_t h[4] = {0, 0, 0, 0};
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "synchronize");
+  e = hipMemcpy(h, ctx->scratch.counters, sizeof(h), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "read nucleation counters");
+  if (wide_events) *wide_events = h[2];
+  if (overflow) *overflow = (int)h[1];
+  if (h[1]) return fail(PF_ERR_OVERFLOW, "nucleation event list overflowed");
+  return PF_OK;
+}
+
+int pf_summarize_f64(pf_ctx* ctx, int nx, int ny, const double* a, const double* b,
+                     const double* c, const double* d, double* out5, void* stream) {
+  return summarize_t<double>(ctx, nx, ny, a, b, c, d, out5, stream);
+}
+
+int pf_summarize_f32(pf_ctx* ctx, int nx, int ny, const float* a, const float* b,
+                     const float* c, const float* d, double* out5, void* stream) {
+  return summarize_t<float>(ctx, nx, ny, a, b, c, d, out5, stream);
+}
+
+int pf_dataviz_f64(int nx, int ny, const double* a, const double* b, const double* c,
+                   const double* d, double* cNi, double* phi, void* stream) {
+  if (int r = check_grid(nx, ny)) return r;
+  if (int r = check_ptrs(a, b, c, d)) return r;
+  if (!cNi || !phi) return fail(PF_ERR_ARG, "null output pointer");
+  hipError_t e = pf::launch_dataviz(nx, ny, a, b, c, d, cNi, phi, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "dataviz kernel");
+  return PF_OK;
+}
+
+int pf_thermo_eval(int what, int n, const double* in, double* out, void* stream) {
+  if (what < 0 || what > 3) return fail(PF_ERR_ARG, "unknown function family");
+  if (n < 0 || (n > 0 && (!in || !out))) return fail(PF_ERR_ARG, "bad sizes or pointers");
+  hipError_t e = pf::launch_thermo_eval(what, n, in, out, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "thermo eval kernel");
+  return PF_OK;
+}
+
+int pf_philox_eval(int n, const uint32_t* in, uint32_t* out, void* stream) {
+  if (n < 0 || (n > 0 && (!in || !out))) return fail(PF_ERR_ARG, "bad sizes or pointers");
+  hipError_t e = pf::launch_philox_eval(n, in, out, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "philox kernel");
+  return PF_OK;
+}
+
+double pf_host_p(double x) { return pf::p(x); }
+double pf_host_pPrime(double x) { return pf::pPrime(x); }
+double pf_host_g(int phase, double xCr, double xNb) {
+  if (phase < 0 || phase > 2) return NAN;
+  return pf::g(phase, xCr, xNb);
+}
+double pf_host_D(int i, int j, double xCr, double xNb) {
+  if (i < 0 || i > 1 || j < 0 || j > 1) return NAN;
+  return pf::D(i, j, xCr, xNb);
+}
+void pf_host_fict(double xCr, double xNb, double pDel, double pGam, double pLav, double* out6) {
+  double mu0, mu1;
+  pf::kks_mu(xCr, xNb, pDel, pGam, pLav, mu0, mu1);
+  for (int a = 0; a < 3; a++) {
+    out6[2 * a] = pf::fict(a, 0, mu0, mu1);
+    out6[2 * a + 1] = pf::fict(a, 1, mu0, mu1);
+  }
+}
+double pf_host_xe(int phase, int comp) {
+  if (phase < 0 || phase > 2 || comp < 0 || comp > 1) return NAN;
+  return pf::XE[phase][comp];
+}
+void pf_host_nucleation_probability(double xCr, double xNb, double dG, double DCrCr,
+                                    double DNbNb, double sigma, double Vatom, double n_gam,
+                                    double dV, double dt, double* Rstar, double* P) {
+  pf::nucleation_probability(xCr, xNb, dG, DCrCr, DNbNb, sigma, Vatom, n_gam, dV, dt, *Rstar,
+                             *P);
+}
+double pf_host_max_effective_diffusivity(const double* DCr, const double* DNb) {
+  // inside pure phase a: dx/dt = D A_gamma H_a lap(x); take the spectral
+  // radius of the 2x2 product for a = gamma, delta, Laves
+  double best = 0.0;
+  for (int a = 0; a < 3; a++) {
+    const double* Ag = pf::A[0];
+    const double* Ha = pf::H[a];
+    // K = A_gamma H_a
+    const double K00 = Ag[0] * Ha[0] + Ag[1] * Ha[1], K01 = Ag[0] * Ha[1] + Ag[1] * Ha[2];
+    const double K10 = Ag[1] * Ha[0] + Ag[2] * Ha[1], K11 = Ag[1] * Ha[1] + Ag[2] * Ha[2];
+    // E = D K
+    const double E00 = DCr[0] * K00 + DCr[1] * K10, E01 = DCr[0] * K01 + DCr[1] * K11;
+    const double E10 = DNb[0] * K00 + DNb[1] * K10, E11 = DNb[0] * K01 + DNb[1] * K11;
+    const double tr = E00 + E11, det = E00 * E11 - E01 * E10;
+    const double disc = tr * tr / 4.0 - det;
+    double lam;
+    if (disc >= 0.0) {
+      const double s = sqrt(disc);
+      lam = fmax(fabs(tr / 2.0 + s), fabs(tr / 2.0 - s));
+    } else {
+      lam = sqrt(fabs(det));  // complex pair: modulus
+    }
+    if (lam > best) best = lam;
+  }
+  return best;
+}
+
+}  // extern "C"

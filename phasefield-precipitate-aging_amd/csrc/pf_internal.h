@@ -1,0 +1,93 @@
+// pf_internal.h -- types shared by the kernels, the C-ABI and the C++ drop-in.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace pf {
+
+// Kinetic and discretisation parameters of one step (by value into kernels).
+struct StepParams {
+  double dt;
+  double DCr0, DCr1, DNb0, DNb1;  // constant diffusivity matrix rows
+  double L0, L1;                  // Allen-Cahn mobilities (delta, Laves)
+  double k0, k1;                  // gradient energy coefficients kappa
+  double w0, w1;                  // double-well heights omega
+  double alpha;                   // three-phase coefficient Omega
+  double m[9];                    // 3x3 Laplacian mask, row-major (y-1, y, y+1)
+};
+
+// Nucleation pass parameters, derived on the host from the reference's
+// device_nucleation arguments (src/discretization.cu:1179-1209).
+struct NucParams {
+  double sigma_del, sigma_lav;
+  double Vatom, n_gam, dV, dt;  // dt = nuc_interval * step dt
+  double dx_arg;                // meshres * nx, as the reference passes it
+  double R_scan;                // 1.75e-9 / dx_arg
+  uint64_t seed, call;
+  uint32_t instance;
+};
+
+// A nucleation event whose embedding cannot be done in place by the
+// deciding thread (embed or scan radius of a cell or more).
+struct Event {
+  uint32_t cell;   // padded index nx*y + x
+  uint32_t phase;  // 0 = delta, 1 = Laves
+  double xCr, xNb, R;
+};
+
+// Device scratch owned by a context: nucleation events and reduction partials.
+struct Scratch {
+  uint32_t* counters;  // [0] event count, [1] overflow flag, [2] wide-event total
+  Event* events;
+  uint32_t event_capacity;
+  double* partials;  // reduction partials: REDUCE_BLOCKS * 10 doubles
+};
+
+constexpr int REDUCE_BLOCKS = 1024;
+constexpr uint32_t EVENT_CAPACITY = 1u << 16;
+
+// Tile configuration of the marching step kernel.
+struct TileConfig {
+  int rows = 0;   // output rows per block (0 = heuristic)
+  int remap = 1;  // XCD-aware block order
+};
+
+hipError_t scratch_alloc(Scratch& s);
+void scratch_free(Scratch& s);
+
+// ---- launchers (pf_step.hip); T = double or float storage, FP64 compute
+template <typename T>
+hipError_t launch_step(int nx, int ny, int bc, const StepParams& P, const T* iCr, const T* iNb,
+                       const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd, T* opl,
+                       const NucParams* nuc, const Scratch& S, const TileConfig& tc,
+                       hipStream_t st);
+template <typename T>
+hipError_t launch_nucleation(int nx, int ny, T* xCr, T* xNb, T* pd, T* pl, const NucParams& N,
+                             const Scratch& S, hipStream_t st);
+template <typename T>
+hipError_t launch_apply_events(int nx, int ny, T* xCr, T* xNb, T* pd, T* pl, const Scratch& S,
+                               hipStream_t st);
+template <typename T>
+hipError_t launch_summarize(int nx, int ny, const T* xCr, const T* xNb, const T* pd,
+                            const T* pl, double* out5, const Scratch& S, hipStream_t st);
+hipError_t launch_dataviz(int nx, int ny, const double* xCr, const double* xNb, const double* pd,
+                          const double* pl, double* conc_Ni, double* phi, hipStream_t st);
+hipError_t launch_thermo_eval(int what, int n, const double* in, double* out, hipStream_t st);
+hipError_t launch_philox_eval(int n, const uint32_t* in, uint32_t* out, hipStream_t st);
+
+int default_rows(int nx, int ny);
+
+NucParams make_nuc(double sigma_del, double sigma_lav, double unit_a, double dx, double dy,
+                   double dz, double dt, uint64_t seed, uint64_t call, uint32_t instance,
+                   int nx, int ny);
+StepParams make_step(double dt, const double* DCr, const double* DNb, const double* kappa,
+                     const double* omega, const double* Lmob, double alpha, const double* mask);
+
+void set_error(const std::string& msg);
+const char* last_error();
+
+}  // namespace pf

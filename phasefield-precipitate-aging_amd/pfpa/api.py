@@ -1,0 +1,258 @@
+"""Python mirror of the reference's host API for the step path.
+
+Same names, argument order and meaning as the C++ functions the driver calls
+(src/main.cpp:236-321):
+
+    make_arrays, five_point_Laplacian_stencil            src/mesh.h:20-79
+    init_cuda, read_out_result, free_cuda               src/data.cuh:16-35
+    device_init_prng, device_boundaries, device_fictitious, fictitious_boundaries,
+    device_laplacian, device_laplacian_boundaries, device_evolution,
+    device_nucleation, device_dataviz                   src/discretization.cuh:39-118
+    swap_pointers_1D                                    src/mesh.h:44
+
+Device memory and streams come from torch (plumbing only); every computation
+runs in libpfpa.so.  Errors raise PfError (the reference asserts in checkCuda,
+src/discretization.cu:23-30).
+
+Differences from the reference, by design:
+  * device_evolution launches one fused kernel that performs the whole step
+    (boundaries, fictitious compositions, Laplacians, Cahn-Hilliard and
+    Allen-Cahn); the five calls before it only record that they were made.
+  * The per-cell cuRAND state (seeded from clock()) is replaced by a
+    counter-based Philox4x32-10 stream: device_init_prng(seed=...) sets the
+    key, and each device_nucleation call advances the stream position.
+  * swap_pointers_1D returns the swapped pair (Python has no pointer-to-pointer).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import PfNucParams, PfParams, check, lib
+
+DEFAULT_SEED = 42
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class HostData:
+    """Host arrays of src/type.h:28-53 (2-D numpy views of 1-D storage)."""
+
+    FIELDS = ("mask_lap", "conc_Ni", "phi", "chem_nrg", "grad_nrg",
+              "conc_Cr_old", "conc_Cr_new", "conc_Cr_lap", "conc_Cr_gam",
+              "conc_Nb_old", "conc_Nb_new", "conc_Nb_lap", "conc_Nb_gam",
+              "phi_del_old", "phi_del_new", "phi_del_lap",
+              "phi_lav_old", "phi_lav_new", "phi_lav_lap")
+
+    def __init__(self):
+        for f in self.FIELDS:
+            setattr(self, f, None)
+
+
+class CudaData:
+    """Device arrays of src/type.h:58-107 that the step path uses.
+
+    conc_*_old/new and phi_*_old/new are torch CUDA tensors (nx*ny FP64, the
+    reference's layout); the driver swaps them between steps."""
+
+    def __init__(self):
+        self.conc_Cr_old = self.conc_Cr_new = None
+        self.conc_Nb_old = self.conc_Nb_new = None
+        self.phi_del_old = self.phi_del_new = None
+        self.phi_lav_old = self.phi_lav_new = None
+        self.conc_Ni = self.phi = None
+        self.ctx = None
+        self.params = None
+        self.bc = _lib.PF_BC_NOFLUX
+        self.seed = DEFAULT_SEED
+        self.nuc_calls = 0
+        self.pending = []
+
+
+def make_arrays(host, nx, ny, nm):
+    """src/mesh.cpp:12-97."""
+    for f in HostData.FIELDS:
+        if f == "mask_lap":
+            host.mask_lap = np.zeros((nm, nm))
+        else:
+            setattr(host, f, np.zeros((ny, nx)))
+
+
+def free_arrays(host):
+    for f in HostData.FIELDS:
+        setattr(host, f, None)
+
+
+def five_point_Laplacian_stencil(dx, dy, mask_lap, nm):
+    """src/mesh.cpp:187-196 (fills mask_lap in place)."""
+    if nm != 3:
+        raise ValueError("five_point_Laplacian_stencil requires nm == 3")
+    mask_lap[0, 1] = 1.0 / (dy * dy)
+    mask_lap[1, 0] = 1.0 / (dx * dx)
+    mask_lap[1, 1] = -2.0 * (dx * dx + dy * dy) / (dx * dx * dy * dy)
+    mask_lap[1, 2] = 1.0 / (dx * dx)
+    mask_lap[2, 1] = 1.0 / (dy * dy)
+
+
+def swap_pointers_1D(a, b):
+    """src/mesh.cpp:163-168."""
+    return b, a
+
+
+def init_cuda(host, nx, ny, nm, kappa, omega, Lmob, D_Cr, D_Nb, dev, bc=_lib.PF_BC_NOFLUX):
+    """src/data.cu:10-92: allocate device fields, upload parameters and the
+    host's *_old fields.  bc selects no-flux (the reference's) or periodic."""
+    if nm != 3:
+        raise ValueError("only 3x3 stencils (nm == 3) are supported")
+    if not torch.cuda.is_available():
+        raise _lib.PfError("init_cuda: no GPU visible")
+    h = ctypes.c_void_p()
+    check(lib().pf_ctx_create(ctypes.byref(h)), "pf_ctx_create")
+    dev.ctx = h
+    dev.bc = int(bc)
+    P = PfParams()
+    for k in range(2):
+        P.DCr[k] = D_Cr[k]
+        P.DNb[k] = D_Nb[k]
+        P.kappa[k] = kappa[k]
+        P.omega[k] = omega[k]
+        P.Lmob[k] = Lmob[k]
+    mask = np.asarray(host.mask_lap, dtype=np.float64).reshape(-1)
+    for k in range(9):
+        P.mask[k] = mask[k]
+    dev.params = P
+    opts = dict(dtype=torch.float64, device="cuda")
+    for name in ("conc_Cr", "conc_Nb", "phi_del", "phi_lav"):
+        src = np.ascontiguousarray(getattr(host, name + "_old"), dtype=np.float64)
+        if src.shape != (ny, nx):
+            raise ValueError("host.%s_old has shape %s, expected %s" % (name, src.shape, (ny, nx)))
+        setattr(dev, name + "_old", torch.from_numpy(src).to("cuda").reshape(-1))
+        setattr(dev, name + "_new", torch.zeros(nx * ny, **opts))
+    dev.conc_Ni = torch.zeros(nx * ny, **opts)
+    dev.phi = torch.zeros(nx * ny, **opts)
+    dev.nx, dev.ny = nx, ny
+
+
+def free_cuda(dev):
+    """src/data.cu:94-154."""
+    if dev.ctx is not None:
+        torch.cuda.synchronize()
+        lib().pf_ctx_destroy(dev.ctx)
+        dev.ctx = None
+    for name in ("conc_Cr", "conc_Nb", "phi_del", "phi_lav"):
+        setattr(dev, name + "_old", None)
+        setattr(dev, name + "_new", None)
+    dev.conc_Ni = dev.phi = None
+
+
+def device_init_prng(dev, nx, ny, nm, bx, by, seed=DEFAULT_SEED):
+    """src/discretization.cu:1033-1044: key the counter-based stream."""
+    dev.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    dev.nuc_calls = 0
+
+
+def _record(dev, name):
+    dev.pending.append(name)
+
+
+def device_boundaries(dev, nx, ny, nm, bx, by):
+    """src/discretization.cu:396-451 -- folded into device_evolution."""
+    _record(dev, "boundaries")
+
+
+def device_fictitious(dev, nx, ny, nm, bx, by):
+    """src/discretization.cu:128-173 -- folded into device_evolution."""
+    _record(dev, "fictitious")
+
+
+def fictitious_boundaries(dev, nx, ny, nm, bx, by):
+    """src/discretization.cu:453-480 -- folded into device_evolution."""
+    _record(dev, "fictitious_boundaries")
+
+
+def device_laplacian(dev, nx, ny, nm, bx, by, dx, dy):
+    """src/discretization.cu:780-841 -- folded into device_evolution (the mask
+    uploaded by init_cuda carries 1/dx^2; dx, dy are unused, as there)."""
+    _record(dev, "laplacian")
+
+
+def device_laplacian_boundaries(dev, nx, ny, nm, bx, by):
+    """src/discretization.cu:524-548 -- folded into device_evolution."""
+    _record(dev, "laplacian_boundaries")
+
+
+def device_evolution(dev, nx, ny, nm, bx, by, alpha, dt):
+    """src/discretization.cu:982-1020: one fused step *_old -> *_new."""
+    dev.pending.clear()
+    P = dev.params
+    P.alpha = alpha
+    P.dt = dt
+    check(lib().pf_step_f64(dev.ctx, nx, ny, dev.bc, ctypes.byref(P),
+                            _ptr(dev.conc_Cr_old), _ptr(dev.conc_Nb_old),
+                            _ptr(dev.phi_del_old), _ptr(dev.phi_lav_old),
+                            _ptr(dev.conc_Cr_new), _ptr(dev.conc_Nb_new),
+                            _ptr(dev.phi_del_new), _ptr(dev.phi_lav_new),
+                            None, _stream()), "device_evolution")
+
+
+def nuc_params(dev, sigma_del, sigma_lav, unit_a, ifce_width, dx, dy, dz, dt):
+    N = PfNucParams()
+    N.sigma_del, N.sigma_lav, N.unit_a, N.ifce_width = sigma_del, sigma_lav, unit_a, ifce_width
+    N.dx, N.dy, N.dz, N.dt = dx, dy, dz, dt
+    N.seed = dev.seed
+    N.call = dev.nuc_calls
+    N.instance = 0
+    return N
+
+
+def device_nucleation(dev, nx, ny, nm, bx, by, sigma_del, sigma_lav, unit_a, ifce_width,
+                      dx, dy, dz, dt):
+    """src/discretization.cu:1179-1209: nucleation pass on the *_new fields."""
+    N = nuc_params(dev, sigma_del, sigma_lav, unit_a, ifce_width, dx, dy, dz, dt)
+    check(lib().pf_nucleation_f64(dev.ctx, nx, ny, _ptr(dev.conc_Cr_new), _ptr(dev.conc_Nb_new),
+                                  _ptr(dev.phi_del_new), _ptr(dev.phi_lav_new),
+                                  ctypes.byref(N), _stream()), "device_nucleation")
+    dev.nuc_calls += 1
+
+
+def read_out_result(dev, host, nx, ny):
+    """src/data.cu:156-172: device *_old -> host *_new (synchronous)."""
+    torch.cuda.synchronize()
+    for name in ("conc_Cr", "conc_Nb", "phi_del", "phi_lav"):
+        getattr(host, name + "_new")[...] = (
+            getattr(dev, name + "_old").reshape(ny, nx).cpu().numpy())
+
+
+def device_dataviz(dev, host, nx, ny, nm, bx, by):
+    """src/discretization.cu:1231-1263."""
+    check(lib().pf_dataviz_f64(nx, ny, _ptr(dev.conc_Cr_old), _ptr(dev.conc_Nb_old),
+                               _ptr(dev.phi_del_old), _ptr(dev.phi_lav_old),
+                               _ptr(dev.conc_Ni), _ptr(dev.phi), _stream()), "device_dataviz")
+    read_out_result(dev, host, nx, ny)
+    host.conc_Ni[...] = dev.conc_Ni.reshape(ny, nx).cpu().numpy()
+    host.phi[...] = dev.phi.reshape(ny, nx).cpu().numpy()
+
+
+def summarize_fields(dev, nx, ny):
+    """Means of {x_Cr, x_Nb, gamma, delta, Laves} over the interior of the
+    latest fields (src/alloy625.cpp:811-863), on the GPU."""
+    out = torch.zeros(5, dtype=torch.float64, device="cuda")
+    check(lib().pf_summarize_f64(dev.ctx, nx, ny, _ptr(dev.conc_Cr_old), _ptr(dev.conc_Nb_old),
+                                 _ptr(dev.phi_del_old), _ptr(dev.phi_lav_old), _ptr(out),
+                                 _stream()), "summarize_fields")
+    return out.cpu().tolist()
+
+
+def nucleation_status(dev):
+    wide = ctypes.c_uint64()
+    over = ctypes.c_int()
+    rc = lib().pf_nucleation_status(dev.ctx, ctypes.byref(wide), ctypes.byref(over))
+    check(rc, "pf_nucleation_status")
+    return wide.value, bool(over.value)

@@ -1,0 +1,110 @@
+"""Native stepping loop over the C-ABI: one fused launch per step, nucleation
+fused into the steps the reference's schedule selects
+(kernel_time % nuc_interval == 0, src/main.cpp:307-316)."""
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import PfNucParams, PfParams, check, lib
+from .model import LATTICE_CONST, IFCE_WIDTH, SIGMA
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Simulation:
+    """Double-buffered fields resident on the GPU.
+
+    dtype torch.float64 is the plain form; torch.float32 the precision form
+    (FP32 storage, FP64 arithmetic)."""
+
+    def __init__(self, setup, fields, dtype=torch.float64, nucleation=False, seed=42,
+                 nuc_every=None, instance=0, stream=None, ctx=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("Simulation needs a GPU")
+        self.s = setup
+        self.dtype = dtype
+        self.nx, self.ny = setup.nx, setup.ny
+        self.stream = stream if stream is not None else torch.cuda.current_stream()
+        self.owns_ctx = ctx is None
+        if ctx is None:
+            h = ctypes.c_void_p()
+            check(lib().pf_ctx_create(ctypes.byref(h)), "pf_ctx_create")
+            ctx = h
+        self.ctx = ctx
+        with torch.cuda.stream(self.stream):
+            self.a = [torch.as_tensor(np.ascontiguousarray(f)).to(device="cuda", dtype=dtype)
+                      for f in fields]
+            self.b = [torch.empty_like(x) for x in self.a]
+        P = PfParams()
+        P.dt = setup.dt
+        for k in range(2):
+            P.DCr[k] = setup.DCr[k]
+            P.DNb[k] = setup.DNb[k]
+            P.kappa[k] = setup.kappa[k]
+            P.omega[k] = setup.omega[k]
+            P.Lmob[k] = setup.Lmob[k]
+        P.alpha = setup.alpha
+        for k in range(9):
+            P.mask[k] = setup.mask[k]
+        self.P = P
+        self.nucleation = nucleation
+        self.nuc_every = nuc_every if nuc_every else max(1, int(0.0001 / setup.dt))
+        N = PfNucParams()
+        N.sigma_del, N.sigma_lav = SIGMA
+        N.unit_a = LATTICE_CONST
+        N.ifce_width = IFCE_WIDTH
+        N.dx = N.dy = N.dz = setup.dx
+        N.dt = self.nuc_every * setup.dt
+        N.seed = seed
+        N.call = 0
+        N.instance = instance
+        self.N = N
+        self.steps_done = 0
+        self._step = lib().pf_step_f64 if dtype == torch.float64 else lib().pf_step_f32
+        self._summ = lib().pf_summarize_f64 if dtype == torch.float64 else lib().pf_summarize_f32
+
+    def set_tile(self, rows, remap=1):
+        check(lib().pf_ctx_set_tile(self.ctx, rows, remap), "pf_ctx_set_tile")
+
+    def step(self, n=1):
+        st = ctypes.c_void_p(self.stream.cuda_stream)
+        for _ in range(n):
+            nuc = None
+            if self.nucleation and self.steps_done % self.nuc_every == 0:
+                self.N.call = self.steps_done // self.nuc_every
+                nuc = ctypes.byref(self.N)
+            a, b = self.a, self.b
+            check(self._step(self.ctx, self.nx, self.ny, self.s.bc, ctypes.byref(self.P),
+                             _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(a[3]),
+                             _ptr(b[0]), _ptr(b[1]), _ptr(b[2]), _ptr(b[3]), nuc, st),
+                  "pf_step")
+            self.a, self.b = self.b, self.a
+            self.steps_done += 1
+
+    def fields(self):
+        self.stream.synchronize()
+        return [x.reshape(self.ny, self.nx).double().cpu().numpy() for x in self.a]
+
+    def summarize(self):
+        out = torch.zeros(5, dtype=torch.float64, device="cuda")
+        a = self.a
+        check(self._summ(self.ctx, self.nx, self.ny, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]),
+                         _ptr(a[3]), _ptr(out), ctypes.c_void_p(self.stream.cuda_stream)),
+              "pf_summarize")
+        self.stream.synchronize()
+        return out.cpu().tolist()
+
+    def close(self):
+        if self.ctx is not None and self.owns_ctx:
+            torch.cuda.synchronize()
+            lib().pf_ctx_destroy(self.ctx)
+        self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,164 @@
+"""CPU-side checks of libpfpa.so: it loads, exports every symbol the public
+headers declare, and its host-side thermodynamics (the same source the
+kernels use) match the reference's golden vectors.  No GPU calls."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import pfpa
+from pfpa import model as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_c_symbols():
+    text = open(os.path.join(ROOT, "include", "pfpa.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pf_\w+)\s*\(", text)))
+
+
+def _exported():
+    out = subprocess.run(["nm", "-D", "--defined-only", pfpa.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if line.strip()}
+
+
+def test_library_loads():
+    L = pfpa.lib()
+    assert L.pf_default_rows(4098, 4098) > 0
+
+
+def test_exports_every_c_symbol():
+    syms = _declared_c_symbols()
+    assert len(syms) >= 20
+    exp = _exported()
+    missing = [s for s in syms if s not in exp]
+    assert not missing, missing
+    L = pfpa.lib()
+    for s in syms:
+        assert getattr(L, s) is not None
+
+
+def test_exports_dropin_cpp_api():
+    """The reference's C++ host API (src/data.cuh, src/discretization.cuh,
+    src/mesh.h) is exported with C++ linkage."""
+    out = subprocess.run(["nm", "-D", "-C", "--defined-only", pfpa.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    for name in ("init_cuda(HostData*", "free_cuda(CudaData*", "read_out_result(CudaData*",
+                 "device_init_prng(CudaData*", "device_boundaries(CudaData*",
+                 "device_fictitious(CudaData*", "fictitious_boundaries(CudaData*",
+                 "device_laplacian(CudaData*", "device_laplacian_boundaries(CudaData*",
+                 "device_evolution(CudaData*", "device_nucleation(CudaData*",
+                 "device_dataviz(CudaData*", "make_arrays(HostData*", "free_arrays(HostData*",
+                 "swap_pointers_1D(double**", "five_point_Laplacian_stencil(double",
+                 "checkCuda(hipError_t", "nTiles(int"):
+        assert name in out, name
+
+
+def test_no_device_calls_fail_cleanly_without_gpu():
+    """Without a GPU the library reports an error instead of crashing."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    L = pfpa.lib()
+    h = ctypes.c_void_p()
+    rc = L.pf_ctx_create(ctypes.byref(h))
+    assert rc != 0
+    assert L.pf_last_error()
+
+
+def test_host_thermo_matches_golden(thermo_golden):
+    G = thermo_golden
+    L = pfpa.lib()
+    for x, p, pp in zip(G["p_in"], G["p_out"], G["pPrime_out"]):
+        assert L.pf_host_p(x) == pytest.approx(p, rel=1e-14, abs=1e-17)
+        assert L.pf_host_pPrime(x) == pytest.approx(pp, rel=1e-14, abs=1e-17)
+    comp = G["comp_in"]
+    for a, ph in enumerate(("gam", "del", "lav")):
+        g = np.array([L.pf_host_g(a, c[0], c[1]) for c in comp])
+        assert np.abs(g - G["g_" + ph]).max() <= 1e-14 * np.abs(G["g_" + ph]).max()
+    scale = np.maximum(np.abs(G["D_CrCr"]), np.abs(G["D_NbNb"]))
+    for (i, j, n) in [(0, 0, "D_CrCr"), (0, 1, "D_CrNb"), (1, 0, "D_NbCr"), (1, 1, "D_NbNb")]:
+        d = np.array([L.pf_host_D(i, j, c[0], c[1]) for c in comp])
+        assert np.all(np.abs(d - G[n]) <= 1e-12 * np.maximum(np.abs(G[n]), 1e-3 * scale)), n
+    fin = G["fict_in"]
+    names = ["fict_%s_%s" % (ph, c) for ph in ("gam", "del", "lav") for c in ("Cr", "Nb")]
+    out = np.zeros((len(fin), 6))
+    buf = (ctypes.c_double * 6)()
+    for r, row in enumerate(fin):
+        L.pf_host_fict(*[float(v) for v in row], buf)
+        out[r] = list(buf)
+    phys = np.abs(fin[:, 2] + fin[:, 3] + fin[:, 4] - 1.0) < 1e-12
+    for k, n in enumerate(names):
+        err = np.abs(out[:, k] - G[n])
+        assert err[phys].max() < 1e-12, n
+        assert np.all(err[~phys] <= 1e-11 * np.maximum(1.0, np.abs(G[n][~phys]))), n
+
+
+def test_host_nucleation_probability_matches_oracle(oracle):
+    L = pfpa.lib()
+    rng = np.random.default_rng(2)
+    R = ctypes.c_double()
+    P = ctypes.c_double()
+    for _ in range(200):
+        xCr, xNb = rng.uniform(0.15, 0.35), rng.uniform(0.01, 0.25)
+        sigma = rng.uniform(0.008, 0.14)
+        dG = oracle.g(0, xCr, xNb)
+        args = (xCr, xNb, dG, oracle.D(0, 0, xCr, xNb), oracle.D(1, 1, xCr, xNb), sigma,
+                0.25 * 0.352e-9 ** 3, 1e9, 1e-20, 10 ** rng.uniform(-3, 9))
+        L.pf_host_nucleation_probability(*args, ctypes.byref(R), ctypes.byref(P))
+        r0, p0 = oracle.nucleation_probability(*args)
+        assert R.value == pytest.approx(r0, rel=1e-14)
+        assert P.value == pytest.approx(p0, rel=1e-12, abs=1e-300)
+
+
+def test_model_parameters():
+    """Diffusivity matrix, mobility and time-step rules at dx = 0.25 nm."""
+    DCr, DNb, L = M.set_diffusion_matrix(0.30, 0.02)
+    assert DCr[0] == pytest.approx(3.962e-18, rel=1e-3)
+    assert DCr[1] == pytest.approx(-3.398e-17, rel=1e-3)
+    assert DNb[0] == pytest.approx(3.056e-17, rel=1e-3)
+    assert DNb[1] == pytest.approx(2.377e-16, rel=1e-3)
+    assert L[0] == pytest.approx(4.0015e-8, rel=1e-4)
+    assert M.kappa()[0] == pytest.approx(4.4318e-10, rel=1e-4)
+    assert M.omega()[0] == pytest.approx(6.864e8, rel=1e-3)
+    dx = 0.25e-9
+    # inside delta the composition diffuses ~58.6x faster than D_NbNb
+    Dmax = M.max_effective_diffusivity(DCr, DNb)
+    assert Dmax == pytest.approx(1.393e-14, rel=2e-3)
+    assert Dmax / DNb[1] == pytest.approx(58.6, rel=2e-3)
+    assert M.dt_stable(dx, DCr, DNb) == pytest.approx(1.12e-7, rel=3e-3)
+    assert M.dt_reference(dx, DCr, DNb, L) == pytest.approx(1.3e-6, rel=1e-12)
+    assert M.nuc_interval(M.dt_stable(dx, DCr, DNb)) == 891
+
+
+def test_max_effective_diffusivity_against_numpy(oracle):
+    """Spectral radius of D H_gamma^-1 H_a over the three phases, by numpy."""
+    DCr, DNb, _ = M.set_diffusion_matrix(0.30, 0.02)
+    Dm = np.array([DCr, DNb])
+    H = [np.array([[oracle.d2g(a, 0), oracle.d2g(a, 1)], [oracle.d2g(a, 1), oracle.d2g(a, 2)]])
+         for a in range(3)]
+    want = max(np.abs(np.linalg.eigvals(Dm @ np.linalg.inv(H[0]) @ H[a])).max() for a in range(3))
+    assert M.max_effective_diffusivity(DCr, DNb) == pytest.approx(want, rel=1e-10)
+
+
+def test_initial_conditions():
+    f = M.init_matrix_with_particles(64, 64, 0.25e-9, 1, seed=3, radius=1.25e-9)
+    xCr, xNb, pd, pl = f
+    assert pd[1:-1, 1:-1].sum() > 0 and pl[1:-1, 1:-1].sum() > 0
+    assert np.all((pd[1:-1, 1:-1] == 0) | (pd[1:-1, 1:-1] == 1))
+    inside = pd[1:-1, 1:-1] == 1
+    assert np.all(xCr[1:-1, 1:-1][inside] == M.lib().pf_host_xe(1, 0))
+    g = M.init_enriched_band(128, 64, 1, seed=42)
+    band = g[0][1:-1, 1:-1] == M.ENRICHED[0]
+    assert band.sum() == 16 * 64
+    mat = ~band
+    assert g[0][1:-1, 1:-1][mat].min() >= M.MATRIX_CR[0]
+    assert g[0][1:-1, 1:-1][mat].max() <= M.MATRIX_CR[1]
+    # ghost ring is the periodic image
+    assert np.array_equal(g[0][0, :], g[0][-2, :])
+    assert np.array_equal(g[0][:, 0], g[0][:, -2])

@@ -1,0 +1,438 @@
+"""Pins of the CPU oracle: golden vectors from the reference's compiled code,
+and what the mathematics fixes (closed forms, invariants, known answers)."""
+import math
+
+import mpmath
+import numpy as np
+import pytest
+
+# ------------------------------------------------------------------ golden vectors
+
+
+def test_constants_match_reference(oracle, thermo_golden):
+    G = thermo_golden
+    assert oracle.kT() == G["const_kT"][0]
+    assert oracle.RT() == G["const_RT"][0]
+    assert oracle.Vm() == G["const_Vm"][0]
+    for a, ph in enumerate(("gam", "del", "lav")):
+        assert oracle.xe(a, 0) == G["const_xe_%s_Cr" % ph][0]
+        assert oracle.xe(a, 1) == G["const_xe_%s_Nb" % ph][0]
+        assert oracle.d2g(a, 0) == G["const_d2g_%s_dxCrCr" % ph][0]
+        assert oracle.d2g(a, 1) == G["const_d2g_%s_dxCrNb" % ph][0]
+        assert oracle.d2g(a, 1) == G["const_d2g_%s_dxNbCr" % ph][0]
+        assert oracle.d2g(a, 2) == G["const_d2g_%s_dxNbNb" % ph][0]
+
+
+def test_p_golden(oracle, thermo_golden):
+    G = thermo_golden
+    for x, p, pp in zip(G["p_in"], G["p_out"], G["pPrime_out"]):
+        # reference host form uses pow(); identical up to rounding
+        assert oracle.p(x) == pytest.approx(p, rel=1e-14, abs=1e-17)
+        assert oracle.pPrime(x) == pytest.approx(pp, rel=1e-14, abs=1e-17)
+
+
+def test_g_dg_golden(oracle, thermo_golden):
+    G = thermo_golden
+    comp = G["comp_in"]
+    for a, ph in enumerate(("gam", "del", "lav")):
+        g = np.array([oracle.g(a, c[0], c[1]) for c in comp])
+        # absolute scale: the paraboloid's typical magnitude over the set
+        scale = np.abs(G["g_" + ph]).max()
+        assert np.abs(g - G["g_" + ph]).max() <= 1e-14 * scale
+        for k, cc in enumerate(("Cr", "Nb")):
+            d = np.array([oracle.dg(a, k, c[0], c[1]) for c in comp])
+            ref = G["dg_%s_dx%s" % (ph, cc)]
+            assert np.abs(d - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+def test_fict_golden(oracle, thermo_golden):
+    """The 2x2 KKS solve reproduces the reference's sympy rational forms."""
+    G = thermo_golden
+    fin = G["fict_in"]
+    names = ["fict_%s_%s" % (ph, c) for ph in ("gam", "del", "lav") for c in ("Cr", "Nb")]
+    out = np.array([oracle.fict(*r) for r in fin])
+    phys = np.abs(fin[:, 2] + fin[:, 3] + fin[:, 4] - 1.0) < 1e-12
+    for k, n in enumerate(names):
+        ref = G[n]
+        err = np.abs(out[:, k] - ref)
+        # physical phase fractions (p sum to one): 1e-12 absolute
+        assert err[phys].max() < 1e-12, n
+        # independent pGam: the solve is as well conditioned as det(M) allows
+        assert np.all(err[~phys] <= 1e-11 * np.maximum(1.0, np.abs(ref[~phys]))), n
+
+
+def test_D_golden(oracle, thermo_golden):
+    G = thermo_golden
+    comp = G["comp_in"]
+    for (i, j, n) in [(0, 0, "D_CrCr"), (0, 1, "D_CrNb"), (1, 0, "D_NbCr"), (1, 1, "D_NbNb")]:
+        d = np.array([oracle.D(i, j, c[0], c[1]) for c in comp])
+        ref = G[n]
+        # the reference sums three mobility terms of either sign; compare on
+        # the scale of the largest entry at each composition
+        scale = np.maximum(np.abs(G["D_CrCr"]), np.abs(G["D_NbNb"]))
+        assert np.all(np.abs(d - ref) <= 1e-12 * np.maximum(np.abs(ref), 1e-3 * scale)), n
+
+
+def test_step_golden_literal_replay(oracle, step_golden):
+    """Oracle vs a literal replay of the reference's kernel sequence (ghost
+    ring, boundary copies, mask convolution, CH/AC kernels) evaluated with the
+    reference's compiled thermodynamics, no-flux, 4 steps."""
+    G = step_golden
+    F = G["fields"]
+    P = oracle.make_params(G["dt"][0], G["DCr"], G["DNb"], G["kappa"], G["omega"], G["Lmob"],
+                           G["alpha"][0], G["mask"])
+    cur = [F[0][k].copy() for k in range(4)]
+    for s in range(1, F.shape[0]):
+        cur = oracle.step(cur, P, 0)
+        for k in range(4):
+            ref = F[s][k][1:-1, 1:-1]
+            a = cur[k][1:-1, 1:-1]
+            sf = np.abs(ref).max()
+            rel = np.abs(a - ref) / np.maximum(np.maximum(np.abs(a), np.abs(ref)), sf)
+            assert rel.max() <= 1e-12, (s, k)
+            # edges of the ghost ring (corners race in the reference)
+            full = np.abs(cur[k] - F[s][k])
+            full[0, 0] = full[0, -1] = full[-1, 0] = full[-1, -1] = 0.0
+            assert full.max() <= 1e-12 * max(sf, 1.0)
+
+
+# ------------------------------------------------------------------ closed forms
+
+
+def test_p_closed_form(oracle):
+    assert oracle.p(0.0) == 0.0
+    assert oracle.p(1.0) == 1.0
+    assert oracle.p(0.5) == 0.5
+    assert oracle.pPrime(0.0) == 0.0 and oracle.pPrime(1.0) == 0.0
+    # p' is the derivative of p
+    for x in np.linspace(-0.2, 1.2, 29):
+        h = 1e-6
+        fd = (oracle.p(x + h) - oracle.p(x - h)) / (2 * h)
+        assert oracle.pPrime(x) == pytest.approx(fd, rel=1e-7, abs=1e-9)
+
+
+def test_paraboloid_properties(oracle):
+    for a in range(3):
+        xe = (oracle.xe(a, 0), oracle.xe(a, 1))
+        assert abs(oracle.g(a, *xe)) < 1e-6
+        assert abs(oracle.dg(a, 0, *xe)) < 1e-3 and abs(oracle.dg(a, 1, *xe)) < 1e-3
+        # Hessian by central differences of the gradient equals d2g
+        h = 1e-4
+        for k, (dc, dn) in enumerate(((h, 0), (0, h))):
+            for comp in range(2):
+                fd = (oracle.dg(a, comp, xe[0] + dc, xe[1] + dn) -
+                      oracle.dg(a, comp, xe[0] - dc, xe[1] - dn)) / (2 * h)
+                want = oracle.d2g(a, k + comp)
+                assert fd == pytest.approx(want, rel=1e-7)
+        # quadratic: g(xe + t v) = t^2 g(xe + v)
+        v = (0.013, -0.007)
+        g1 = oracle.g(a, xe[0] + v[0], xe[1] + v[1])
+        g3 = oracle.g(a, xe[0] + 3 * v[0], xe[1] + 3 * v[1])
+        assert g3 == pytest.approx(9 * g1, rel=1e-10)
+
+
+def test_fict_invariants(oracle):
+    rng = np.random.default_rng(3)
+    for _ in range(300):
+        xCr, xNb = rng.uniform(0.05, 0.5), rng.uniform(0.01, 0.25)
+        pd = oracle.p(rng.uniform(0, 1))
+        pl = oracle.p(rng.uniform(0, 1)) * (1 - pd)
+        pg = 1 - pd - pl
+        f = oracle.fict(xCr, xNb, pd, pg, pl)
+        # mass balance
+        assert pg * f[0] + pd * f[2] + pl * f[4] == pytest.approx(xCr, abs=1e-14)
+        assert pg * f[1] + pd * f[3] + pl * f[5] == pytest.approx(xNb, abs=1e-14)
+        # equal diffusion potentials across phases
+        mu = [(oracle.dg(a, 0, f[2 * a], f[2 * a + 1]), oracle.dg(a, 1, f[2 * a], f[2 * a + 1]))
+              for a in range(3)]
+        for a in (1, 2):
+            assert mu[a][0] == pytest.approx(mu[0][0], rel=1e-9, abs=1e-2)
+            assert mu[a][1] == pytest.approx(mu[0][1], rel=1e-9, abs=1e-2)
+    # pure gamma: x_gamma = x
+    f = oracle.fict(0.3, 0.02, 0.0, 1.0, 0.0)
+    assert f[0] == pytest.approx(0.3, abs=1e-15) and f[1] == pytest.approx(0.02, abs=1e-15)
+
+
+def test_fict_jacobian_in_delta(oracle):
+    """Inside pure delta, dx_gamma/dx = H_gamma^-1 H_delta; its entries are the
+    numbers that follow from the reference's rational forms."""
+    x0 = (0.05, 0.24)
+    h = 1e-6
+    J = np.zeros((2, 2))
+    for j, (dc, dn) in enumerate(((h, 0), (0, h))):
+        fp = oracle.fict(x0[0] + dc, x0[1] + dn, 1.0, 0.0, 0.0)
+        fm = oracle.fict(x0[0] - dc, x0[1] - dn, 1.0, 0.0, 0.0)
+        J[0, j] = (fp[0] - fm[0]) / (2 * h)
+        J[1, j] = (fp[1] - fm[1]) / (2 * h)
+    want = np.array([[16.349, -334.807], [-2.0826, 101.687]])
+    assert np.allclose(J, want, rtol=2e-4)
+
+
+def _uniform_fields(nx, ny, xCr, xNb, pd, pl):
+    return [np.full((ny, nx), v) for v in (xCr, xNb, pd, pl)]
+
+
+def _params(oracle, dt, dx=0.25e-9):
+    DCr, DNb, L = oracle.set_diffusion_matrix(0.30, 0.02, 2.5e-9)
+    kappa = [3.0 * 0.13 * 2.5e-9 / 2.2] * 2
+    omega = [18.0 * 0.13 * 0.13 / kappa[0]] * 2
+    return oracle.make_params(dt, DCr, DNb, kappa, omega, L, 1.07e11,
+                              oracle.five_point_mask(dx, dx))
+
+
+def test_fixed_points(oracle):
+    P = _params(oracle, 1e-7)
+    for bc in (0, 1):
+        f = _uniform_fields(9, 7, 0.3, 0.02, 0.0, 0.0)
+        out = oracle.step(f, P, bc)
+        for a, b in zip(out, f):
+            assert np.array_equal(a, b)
+        f = _uniform_fields(9, 7, oracle.xe(1, 0), oracle.xe(1, 1), 1.0, 0.0)
+        out = oracle.step(f, P, bc)
+        for a, b in zip(out, f):
+            assert np.abs(a - b).max() < 1e-14
+
+
+def test_laplacian_quadratic_and_fourier(oracle):
+    dx = 0.25e-9
+    dt = 1e-7
+    P = _params(oracle, dt, dx)
+    # quadratic in x and y, pure gamma (x_gamma = x): CH increment is
+    # dt * D . (2a + 2b) away from the boundary
+    ny, nx = 12, 14
+    yy, xx = np.mgrid[0:ny, 0:nx].astype(float)
+    a, b, c, d = 2e-5, -1e-5, 3e-6, 4e-6
+    xCr = 0.3 + a * xx ** 2 + b * yy ** 2
+    xNb = 0.02 + c * xx ** 2 + d * yy ** 2
+    z = np.zeros_like(xCr)
+    out = oracle.step([xCr, xNb, z, z], P, 0)
+    lapCr = 2 * (a + b) / dx ** 2
+    lapNb = 2 * (c + d) / dx ** 2
+    want = dt * (P.DCr[0] * lapCr + P.DCr[1] * lapNb)
+    inc = (out[0] - xCr)[2:-2, 2:-2]
+    assert np.allclose(inc, want, rtol=1e-6, atol=0)
+    # Fourier mode, periodic: eigenvalue -(4/dx^2) sin^2(k dx / 2)
+    Nx, Ny = 16, 6
+    k = 2 * math.pi * 3 / (Nx * dx)
+    col = np.arange(Nx)
+    eps = 1e-4
+    row = 0.3 + eps * np.cos(k * col * dx)
+    xCr = np.zeros((Ny + 2, Nx + 2))
+    xCr[1:-1, 1:-1] = row
+    xNb = np.full_like(xCr, 0.02)
+    z = np.zeros_like(xCr)
+    out = oracle.step([xCr, xNb, z, z], P, 1)
+    lam = -(4 / dx ** 2) * math.sin(k * dx / 2) ** 2
+    want = dt * P.DCr[0] * lam * eps * np.cos(k * col * dx)
+    inc = (out[0] - xCr)[1:-1, 1:-1]
+    # the increment is recovered next to 0.3: one ulp of 0.3 is 5.6e-17
+    assert np.allclose(inc, want[None, :], rtol=1e-10, atol=6e-17)
+
+
+def test_mass_conservation(oracle):
+    import pfpa.model as M  # noqa: F401  (imports the host package; no GPU needed)
+    rng = np.random.default_rng(5)
+    P = _params(oracle, 2e-7)
+    for bc in (0, 1):
+        ny, nx = 22, 26
+        xCr = 0.3 + 1e-2 * rng.standard_normal((ny, nx))
+        xNb = 0.02 + 5e-3 * rng.standard_normal((ny, nx))
+        pd = np.clip(rng.uniform(-0.2, 1.2, (ny, nx)), 0, 1) * (rng.uniform(size=(ny, nx)) > 0.7)
+        pl = np.zeros((ny, nx))
+        f = [xCr, xNb, pd, pl]
+        m0 = [f[0][1:-1, 1:-1].sum(), f[1][1:-1, 1:-1].sum()]
+        out = oracle.run(f, P, bc, 20)
+        m1 = [out[0][1:-1, 1:-1].sum(), out[1][1:-1, 1:-1].sum()]
+        assert m1[0] == pytest.approx(m0[0], rel=1e-13)
+        assert m1[1] == pytest.approx(m0[1], rel=1e-13)
+
+
+def test_free_energy_decreases(oracle):
+    """Allen-Cahn with no chemical driving force is a gradient flow: the
+    discrete double-well + gradient energy must not increase."""
+    dx = 0.25e-9
+    P = _params(oracle, 1e-6, dx)
+    rng = np.random.default_rng(9)
+    ny, nx = 18, 18
+    pd = np.clip(0.5 + 0.4 * rng.standard_normal((ny, nx)), 0, 1)
+    # x at delta equilibrium where phi is 1, gamma equilibrium elsewhere makes
+    # mu = 0 only for pure cells; use pure-gamma compositions and zero mobility
+    # of composition by taking x = xe_gamma everywhere and phi small
+    xCr = np.full((ny, nx), oracle.xe(0, 0))
+    xNb = np.full((ny, nx), oracle.xe(0, 1))
+    pl = np.zeros((ny, nx))
+
+    def energy(f):
+        phi = f[2][1:-1, 1:-1]
+        ext = f[2]
+        gx = (ext[1:-1, 2:] - ext[1:-1, 1:-1]) / dx
+        gy = (ext[2:, 1:-1] - ext[1:-1, 1:-1]) / dx
+        w = P.omega[0] * phi ** 2 * (1 - phi) ** 2
+        return float((w + 0.5 * P.kappa[0] * (gx ** 2 + gy ** 2)).sum())
+
+    f = [xCr, xNb, pd, pl]
+    e_prev = energy(f)
+    for _ in range(30):
+        f = oracle.step(f, P, 1)
+        e = energy(f)
+        assert e <= e_prev * (1 + 1e-12)
+        e_prev = e
+
+
+# ------------------------------------------------------------------ nucleation
+
+
+def test_nucleation_probability_textbook(oracle):
+    """Classical nucleation theory in its textbook form, evaluated at 40
+    digits: R* = 2 sigma/dG, G* = 4/3 pi sigma R*^2,
+    Z = V_a/(2 pi R*^2) sqrt(sigma/kT), beta*_i = 4 pi R*^2 D_ii x_i / a^4,
+    J_i = beta*_i Z n exp(-G*/(kT dc_i)), P = 1 - exp(-dt dV sum J)."""
+    mpmath.mp.dps = 40
+    kT = mpmath.mpf(oracle.kT())
+    a = mpmath.mpf("0.352e-9")
+    xeCr, xeNb = mpmath.mpf(oracle.xe(0, 0)), mpmath.mpf(oracle.xe(0, 1))
+    cases = [
+        # xCr, xNb, sigma, dt, dV scale
+        (0.30, 0.20, 0.01, 1e-4, 1.0),
+        (0.25, 0.22, 0.012, 1e-3, 1.0),
+        (0.20, 0.23, 0.008, 1e-6, 1.0),
+        (0.30, 0.02, 0.13, 1e-4, 1.0),     # matrix: underflows to P = 0
+    ]
+    for (xCr, xNb, sigma, dt, _) in cases:
+        dG = oracle.g(0, xCr, xNb)
+        DCrCr = oracle.D(0, 0, xCr, xNb)
+        DNbNb = oracle.D(1, 1, xCr, xNb)
+        Vatom = 0.25 * 0.352e-9 ** 3
+        dV = (0.25e-9 * 64) ** 2 * 0.25e-9
+        n_gam = dV / Vatom
+        Rs, Pn = oracle.nucleation_probability(xCr, xNb, dG, DCrCr, DNbNb, sigma, Vatom, n_gam,
+                                               dV, dt)
+        s = mpmath.mpf(sigma)
+        R = 2 * s / mpmath.mpf(dG)
+        Gs = mpmath.mpf(4) / 3 * mpmath.pi * s * R ** 2
+        Z = mpmath.mpf(Vatom) / (2 * mpmath.pi * R ** 2) * mpmath.sqrt(s / kT)
+        J = 0
+        for x, D, dc in ((xCr, DCrCr, xeCr - mpmath.mpf(xCr)), (xNb, DNbNb, mpmath.mpf(xNb) - xeNb)):
+            beta = 4 * mpmath.pi * R ** 2 * mpmath.mpf(D) * mpmath.mpf(x) / a ** 4
+            J += beta * Z * mpmath.mpf(n_gam) * mpmath.exp(-Gs / (kT * dc))
+        lam = mpmath.mpf(dt) * mpmath.mpf(dV) * J
+        assert Rs == pytest.approx(float(R), rel=1e-14)
+        # the reference forms 1 - exp(-lambda) in FP64 (rounds to 0 below ~1e-16)
+        assert Pn == pytest.approx(1.0 - math.exp(-float(lam)), rel=1e-9, abs=1e-300)
+        if lam > 1e-6:
+            assert Pn == pytest.approx(float(1 - mpmath.exp(-lam)), rel=1e-9)
+
+
+def _rate(oracle, xCr, xNb, sigma, Vatom, n_gam, dV):
+    """lambda / dt = dV (J_Cr + J_Nb), recovered from P at a small dt."""
+    dG = oracle.g(0, xCr, xNb)
+    args = (xCr, xNb, dG, oracle.D(0, 0, xCr, xNb), oracle.D(1, 1, xCr, xNb), sigma, Vatom,
+            n_gam, dV)
+    _, P = oracle.nucleation_probability(*args, 1.0)
+    return -math.log1p(-P) if P < 1 else float("inf")
+
+
+def test_nucleation_poisson_law(oracle):
+    """P over 2 dt is the probability of at least one event in two
+    independent intervals: 1 - (1 - P(dt))^2."""
+    xCr, xNb, sigma = 0.27, 0.21, 0.011
+    Vatom = 0.25 * 0.352e-9 ** 3
+    dV = 1e-5 * 1e-5 * 0.25e-9
+    n_gam = dV / Vatom
+    rate = _rate(oracle, xCr, xNb, sigma, Vatom, n_gam, dV)
+    assert rate > 0
+    dt = 0.5 / rate
+    dG = oracle.g(0, xCr, xNb)
+    args = (xCr, xNb, dG, oracle.D(0, 0, xCr, xNb), oracle.D(1, 1, xCr, xNb), sigma, Vatom,
+            n_gam, dV)
+    _, P1 = oracle.nucleation_probability(*args, dt)
+    _, P2 = oracle.nucleation_probability(*args, 2 * dt)
+    assert 0.2 < P1 < 0.8
+    assert P2 == pytest.approx(1 - (1 - P1) ** 2, rel=1e-12)
+
+
+def test_philox_known_answers(oracle):
+    """Random123 known-answer vectors for Philox4x32-10."""
+    assert oracle.philox([0, 0, 0, 0], [0, 0]) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    ff = 0xffffffff
+    assert oracle.philox([ff] * 4, [ff, ff]) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert oracle.philox([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344],
+                         [0xa4093822, 0x299f31d0]) == [0xd16cfe09, 0x94fdcceb, 0x5001e420,
+                                                       0x24126ea1]
+
+
+def test_uniforms_in_unit_interval(oracle):
+    us = [oracle.nuc_uniforms(42, call, 0, cell) for call in range(3) for cell in range(2000)]
+    u = np.array(us).ravel()
+    assert u.min() > 0.0 and u.max() <= 1.0
+    assert abs(u.mean() - 0.5) < 0.01
+
+
+def synthetic_nucleation_case(oracle, nx=40, ny=34, seed=11):
+    """Fields and nucleation arguments for which P lies inside (0, 1) on most
+    cells (at the benchmark's own parameters P rounds to 0).  The domain is
+    given a large mesh spacing and the pass a long interval."""
+    rng = np.random.default_rng(seed)
+    xCr = rng.uniform(0.15, 0.3, (ny, nx))
+    xNb = rng.uniform(0.19, 0.24, (ny, nx))
+    pd = np.zeros((ny, nx))
+    pl = np.zeros((ny, nx))
+    pd[10:14, 10:14] = 0.5      # existing precipitate: no nucleation there
+    pl[20:22, 5:9] = 1e-3       # p(phi) ~ 1e-8 > 1e-10: also excluded
+    dx = 2.5e-7
+    sig = (0.0105, 0.0108)
+    dV = dx * nx * dx * ny * dx
+    Vatom = 0.25 * 0.352e-9 ** 3
+    rate = _rate(oracle, 0.225, 0.215, sig[0], Vatom, dV / Vatom, dV)
+    dt_n = 0.7 / rate
+    return [xCr, xNb, pd, pl], dict(sigma_del=sig[0], sigma_lav=sig[1], unit_a=0.352e-9,
+                                    dx=dx, dz=dx, dt=dt_n)
+
+
+def test_nucleation_decisions_and_embed(oracle):
+    """Synthetic fields where P is in (0, 1): the event flags equal the
+    decision rule applied cell by cell, and each event embeds the phase's
+    equilibrium composition at its centre (delta before Laves)."""
+    fields, a = synthetic_nucleation_case(oracle)
+    xCr, xNb, pd, pl = fields
+    ny, nx = xCr.shape
+    f, nev, flags = oracle.nucleation(fields, a["sigma_del"], a["sigma_lav"], a["unit_a"],
+                                      a["dx"] * nx, a["dx"] * ny, a["dz"], a["dt"], seed=7,
+                                      call=3)
+    assert 0 < nev < 2 * (nx - 2) * (ny - 2)
+    dV = a["dx"] * nx * a["dx"] * ny * a["dz"]
+    Vatom = 0.25 * a["unit_a"] ** 3
+    seen = 0
+    probs = []
+    for y in range(1, ny - 1):
+        for x in range(1, nx - 1):
+            idx = nx * y + x
+            want = 0
+            if oracle.p(pd[y, x]) + oracle.p(pl[y, x]) < 1e-10:
+                u = oracle.nuc_uniforms(7, 3, 0, idx)
+                dG = oracle.g(0, xCr[y, x], xNb[y, x])
+                for ph in range(2):
+                    R, P = oracle.nucleation_probability(
+                        xCr[y, x], xNb[y, x], dG, oracle.D(0, 0, xCr[y, x], xNb[y, x]),
+                        oracle.D(1, 1, xCr[y, x], xNb[y, x]),
+                        (a["sigma_del"], a["sigma_lav"])[ph], Vatom, dV / Vatom, dV, a["dt"])
+                    probs.append(P)
+                    if R > 0 and P - u[ph] > 0:
+                        want |= 1 << ph
+            assert flags[y, x] == want
+            if want:
+                seen += 1
+                last = 2 if want & 2 else 1
+                assert f[0][y, x] == oracle.xe(last, 0)
+                assert f[1][y, x] == oracle.xe(last, 1)
+                if want & 1:
+                    assert f[2][y, x] == 1.0
+                if want & 2:
+                    assert f[3][y, x] == 1.0
+            else:
+                for k in range(4):
+                    assert f[k][y, x] == fields[k][y, x]
+    probs = np.array(probs)
+    assert ((probs > 0.05) & (probs < 0.95)).mean() > 0.5
+    assert seen > 100
+    assert flags[10:14, 10:14].sum() == 0 and flags[20:22, 5:9].sum() == 0
