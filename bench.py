@@ -62,9 +62,25 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--form", choices=["plain", "precision"], default="plain")
     ap.add_argument("--size", type=int, default=4096)
-    ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a plain device copy of 4 x 4098^2 FP64 arrays (1.07 GB moved per "
+                         "dispatch) to scale rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        n = 4098 * 4098
+        src = [torch.ones(n, dtype=torch.float64, device="cuda") for _ in range(4)]
+        dst = [torch.empty_like(x) for x in src]
+        for _ in range(5):
+            for a, b in zip(dst, src):
+                a.copy_(b)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "torch copy_ of 4 x %d FP64", "bytes_read_per_dispatch":
+                          n * 8, "bytes_written_per_dispatch": n * 8}))
+        return
 
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "

@@ -67,8 +67,10 @@ int pf_ctx_create(pf_ctx** ctx);
 void pf_ctx_destroy(pf_ctx* ctx);
 const char* pf_last_error(void);
 int pf_device_count(void);
-/* rows per block of the marching step kernel (0 = heuristic), XCD-aware block order on/off */
-int pf_ctx_set_tile(pf_ctx* ctx, int rows, int xcd_remap);
+/* Tile of the marching step kernel: output rows per block (0 = heuristic),
+ * XCD-aware block order on/off, rows of loads kept in flight (2, 3 or 6;
+ * 0 = default 3).  Results do not depend on the tile. */
+int pf_ctx_set_tile(pf_ctx* ctx, int rows, int xcd_remap, int depth);
 
 /* One fused explicit-Euler step, old -> new.  Replaces the per-step sequence
  * device_boundaries, device_fictitious, fictitious_boundaries,

@@ -141,11 +141,14 @@ void pf_ctx_destroy(pf_ctx* ctx) {
 
 const char* pf_last_error(void) { return pf::last_error(); }
 
-int pf_ctx_set_tile(pf_ctx* ctx, int rows, int xcd_remap) {
+int pf_ctx_set_tile(pf_ctx* ctx, int rows, int xcd_remap, int depth) {
   if (!ctx) return fail(PF_ERR_ARG, "null context");
   if (rows < 0 || rows > 1 << 20) return fail(PF_ERR_ARG, "rows out of range");
+  if (depth != 0 && depth != 2 && depth != 3 && depth != 6)
+    return fail(PF_ERR_ARG, "prefetch depth must be 2, 3 or 6 (0 = default)");
   ctx->tile.rows = rows;
   ctx->tile.remap = xcd_remap ? 1 : 0;
+  ctx->tile.depth = depth ? depth : 3;
   return PF_OK;
 }
 

@@ -54,6 +54,7 @@ constexpr uint32_t EVENT_CAPACITY = 1u << 16;
 struct TileConfig {
   int rows = 0;   // output rows per block (0 = heuristic)
   int remap = 1;  // XCD-aware block order
+  int depth = 3;  // rows of loads in flight ahead of the computed row (2, 3 or 6)
 };
 
 hipError_t scratch_alloc(Scratch& s);

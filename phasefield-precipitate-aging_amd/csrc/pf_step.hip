@@ -7,13 +7,16 @@
 //
 // Work decomposition ("marching strips"): one wave64 per block owns a strip of
 // 64 columns (62 output columns + one halo column each side) and marches down
-// a chunk of rows.  Each lane keeps a three-row window (y-1, y, y+1) of the
-// quantities the 5-point stencil needs -- phi_delta, phi_Laves and the
-// fictitious gamma compositions -- in registers, so every cell is loaded once
-// per strip and its KKS solve is done once; x-neighbours come from adjacent
-// lanes by shuffles.  Row y+2 is prefetched while row y is computed.  Blocks
-// are renumbered so that each of the 8 XCDs (each with its own L2) works on a
-// contiguous band of the domain, keeping halo re-reads inside one L2.
+// a chunk of rows.  Each lane keeps a three-row window (y-1, y, y+1) of what
+// the stencil needs -- phi_delta, phi_Laves and the fictitious gamma
+// composition -- in registers, so each cell is loaded once per strip and its
+// KKS solve is done once.  x-neighbours come from adjacent lanes through DPP
+// wave shifts (no LDS round trip).  Loads run PF_DEPTH rows ahead of the row
+// being computed (a register ring), so a wave keeps several rows of HBM
+// requests in flight; the row loop is unrolled by 6 so the window and the
+// ring rotate by register renaming instead of moves.  Blocks are renumbered so
+// that each of the 8 XCDs (each with its own L2) works on a contiguous band of
+// the domain, keeping halo re-reads inside one L2.
 //
 // Boundary conditions are index maps (no ghost reads): no-flux clamps to the
 // nearest interior cell, which is exactly the reference's ghost copy
@@ -32,6 +35,8 @@ namespace pf {
 
 constexpr int WAVE = 64;
 constexpr int STRIP = WAVE - 2;  // output columns per strip
+constexpr int WPB = 4;           // waves (strips) per block: adjacent strips of one row chunk
+constexpr int UNROLL = 6;        // multiple of the 3-row window and of every ring depth
 
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
   // Hardware dispatches block b to XCD b % 8.  Give XCD x the contiguous
@@ -50,6 +55,19 @@ __device__ __forceinline__ int bc_map(int i, int n) {
   return i;
 }
 
+// DPP wave shifts (GFX9 wave_shr:1 / wave_shl:1): lane i receives lane i-1 /
+// lane i+1.  Lanes 0 and 63 receive 0; their results are never stored.
+__device__ __forceinline__ double from_left(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double from_right(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
 struct Point {       // what the neighbours of a cell need
   double pd, pl;     // phase fields
   double gCr, gNb;   // fictitious gamma composition
@@ -58,13 +76,17 @@ struct Centre {      // extra state of a cell when it is the stencil centre
   double xCr, xNb;   // compositions
   double mu0, mu1;   // common diffusion potential
 };
+template <typename T>
+struct Raw {
+  T cr, nb, pd, pl;
+};
 
 template <typename T>
-__device__ __forceinline__ void derive(T rCr, T rNb, T rpd, T rpl, Point& v, Centre& c) {
-  v.pd = (double)rpd;
-  v.pl = (double)rpl;
-  c.xCr = (double)rCr;
-  c.xNb = (double)rNb;
+__device__ __forceinline__ void derive(const Raw<T>& r, Point& v, Centre& c) {
+  v.pd = (double)r.pd;
+  v.pl = (double)r.pl;
+  c.xCr = (double)r.cr;
+  c.xNb = (double)r.nb;
   const double pD = p(v.pd);
   const double pL = p(v.pl);
   const double pG = 1.0 - pD - pL;
@@ -73,15 +95,19 @@ __device__ __forceinline__ void derive(T rCr, T rNb, T rpd, T rpl, Point& v, Cen
   v.gNb = fict(0, 1, c.mu0, c.mu1);
 }
 
-__device__ __forceinline__ double shfl_up1(double v) { return __shfl_up(v, 1, WAVE); }
-__device__ __forceinline__ double shfl_dn1(double v) { return __shfl_down(v, 1, WAVE); }
+__device__ __forceinline__ Point shift_left(const Point& a) {
+  return Point{from_left(a.pd), from_left(a.pl), from_left(a.gCr), from_left(a.gNb)};
+}
+__device__ __forceinline__ Point shift_right(const Point& a) {
+  return Point{from_right(a.pd), from_right(a.pl), from_right(a.gCr), from_right(a.gNb)};
+}
 
 // Nucleation of one cell from its new values (centre-only scan; the caller
 // guarantees R_scan < 1).  Mirrors src/discretization.cu:1104-1176 with the
 // event order delta then Laves on the pre-embed composition.
 __device__ __forceinline__ void nucleate_cell(const NucParams& N, uint32_t cell, double& xCr,
-                                              double& xNb, double& pd, double& pl,
-                                              const Scratch& S) {
+                                           double& xNb, double& pd, double& pl,
+                                           const Scratch& S) {
   const double phi_pre = p(pd) + p(pl);
   if (!(phi_pre < 1e-10)) return;
   double u1, u2;
@@ -121,15 +147,68 @@ __device__ __forceinline__ void nucleate_cell(const NucParams& N, uint32_t cell,
   }
 }
 
-template <typename T, int BC, bool NINE, bool NUC>
-__global__ __launch_bounds__(WAVE) void step_kernel(
+// Laplacians of the four stencil quantities in the reference's mask order
+// (src/discretization.cu:589-595): rows y-1, y, y+1; columns x-1, x, x+1.
+template <bool NINE>
+__device__ __forceinline__ void laplacians(const double* m, const Point& S, const Point& W,
+                                           const Point& C, const Point& E, const Point& Nn,
+                                           const Point& SW, const Point& SE, const Point& NW,
+                                           const Point& NE, double lap[4]) {
+  const double s[4] = {S.pd, S.pl, S.gCr, S.gNb};
+  const double w[4] = {W.pd, W.pl, W.gCr, W.gNb};
+  const double c[4] = {C.pd, C.pl, C.gCr, C.gNb};
+  const double e[4] = {E.pd, E.pl, E.gCr, E.gNb};
+  const double n[4] = {Nn.pd, Nn.pl, Nn.gCr, Nn.gNb};
+#pragma unroll
+  for (int f = 0; f < 4; f++) {
+    double v;
+    if (NINE) {
+      const double sw[4] = {SW.pd, SW.pl, SW.gCr, SW.gNb};
+      const double se[4] = {SE.pd, SE.pl, SE.gCr, SE.gNb};
+      const double nw[4] = {NW.pd, NW.pl, NW.gCr, NW.gNb};
+      const double ne[4] = {NE.pd, NE.pl, NE.gCr, NE.gNb};
+      v = m[0] * sw[f];
+      v = fma(m[1], s[f], v);
+      v = fma(m[2], se[f], v);
+      v = fma(m[3], w[f], v);
+      v = fma(m[4], c[f], v);
+      v = fma(m[5], e[f], v);
+      v = fma(m[6], nw[f], v);
+      v = fma(m[7], n[f], v);
+      v = fma(m[8], ne[f], v);
+    } else {
+      v = m[1] * s[f];
+      v = fma(m[3], w[f], v);
+      v = fma(m[4], c[f], v);
+      v = fma(m[5], e[f], v);
+      v = fma(m[7], n[f], v);
+    }
+    lap[f] = v;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void store4(T* oCr, T* oNb, T* opd, T* opl, size_t o, double a,
+                                       double b, double c, double d) {
+  oCr[o] = (T)a;
+  oNb[o] = (T)b;
+  opd[o] = (T)c;
+  opl[o] = (T)d;
+}
+
+template <typename T, int BC, bool NINE, bool NUC, int DEPTH>
+__global__ __launch_bounds__(WAVE * WPB) void step_kernel(
     const T* __restrict__ iCr, const T* __restrict__ iNb, const T* __restrict__ ipd,
     const T* __restrict__ ipl, T* __restrict__ oCr, T* __restrict__ oNb, T* __restrict__ opd,
     T* __restrict__ opl, int Nx, int Ny, int pitch, int strips, int chunks, int rows, int remap,
     StepParams P, NucParams N, Scratch S) {
-  const int lane = threadIdx.x;
-  const int nb = strips * chunks;
-  const int b = remap ? xcd_remap(blockIdx.x, nb) : (int)blockIdx.x;
+  static_assert(UNROLL % DEPTH == 0, "ring depth must divide the unroll factor");
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int nwaves = strips * chunks;
+  const int nblocks = (nwaves + WPB - 1) / WPB;
+  const int b = (remap ? xcd_remap(blockIdx.x, nblocks) : (int)blockIdx.x) * WPB +
+                (int)(threadIdx.x / WAVE);
+  if (b >= nwaves) return;  // whole wave exits; no barriers in this kernel
   const int strip = b % strips;
   const int chunk = b / strips;
 
@@ -144,158 +223,111 @@ __global__ __launch_bounds__(WAVE) void step_kernel(
   auto row_base = [&](int y) -> size_t {
     return (size_t)(1 + bc_map<BC>(y, Ny)) * (size_t)pitch + (size_t)px;
   };
+  auto load = [&](Raw<T>& r, int y) {
+    const size_t a = row_base(y);
+    r.cr = iCr[a];
+    r.nb = iNb[a];
+    r.pd = ipd[a];
+    r.pl = ipl[a];
+  };
 
-  // window: rows y-1 (prev), y (cur), y+1 (next)
-  Point prv, cur, nxt;
-  Centre ccur, cnxt, cdump;
+  // window slots (row y0 + k - 1 + 3j lives in slot k % 3 pattern below)
+  Point win[3];
+  Centre cen[3];
+  Point winW[3], winE[3];  // DPP-shifted neighbours (nine-point masks only)
+  Raw<T> ring[DEPTH];
+
   {
-    const size_t a = row_base(y0 - 1);
-    derive<T>(iCr[a], iNb[a], ipd[a], ipl[a], prv, cdump);
-    const size_t c = row_base(y0);
-    derive<T>(iCr[c], iNb[c], ipd[c], ipl[c], cur, ccur);
+    Raw<T> r;
+    load(r, y0 - 1);
+    derive<T>(r, win[2], cen[2]);
+    load(r, y0);
+    derive<T>(r, win[0], cen[0]);
   }
-  T rCr, rNb, rpd, rpl;  // raw loads of the next row, in flight during compute
-  {
-    const size_t a = row_base(y0 + 1);
-    rCr = iCr[a];
-    rNb = iNb[a];
-    rpd = ipd[a];
-    rpl = ipl[a];
-  }
-  // diagonal neighbours for a nine-point mask
-  Point prvW, prvE;
+#pragma unroll
+  for (int j = 0; j < DEPTH; j++) load(ring[j], min(y0 + 1 + j, y1));
   if (NINE) {
-    prvW.pd = shfl_up1(prv.pd); prvW.pl = shfl_up1(prv.pl);
-    prvW.gCr = shfl_up1(prv.gCr); prvW.gNb = shfl_up1(prv.gNb);
-    prvE.pd = shfl_dn1(prv.pd); prvE.pl = shfl_dn1(prv.pl);
-    prvE.gCr = shfl_dn1(prv.gCr); prvE.gNb = shfl_dn1(prv.gNb);
+    winW[2] = shift_left(win[2]);
+    winE[2] = shift_right(win[2]);
+    winW[0] = shift_left(win[0]);
+    winE[0] = shift_right(win[0]);
   }
 
   const double* m = P.m;
-  for (int y = y0; y < y1; y++) {
-    derive<T>(rCr, rNb, rpd, rpl, nxt, cnxt);
-    if (y + 1 < y1) {
-      const size_t a = row_base(y + 2);
-      rCr = iCr[a];
-      rNb = iNb[a];
-      rpd = ipd[a];
-      rpl = ipl[a];
-    }
-    Point W, E;
-    W.pd = shfl_up1(cur.pd); W.pl = shfl_up1(cur.pl);
-    W.gCr = shfl_up1(cur.gCr); W.gNb = shfl_up1(cur.gNb);
-    E.pd = shfl_dn1(cur.pd); E.pl = shfl_dn1(cur.pl);
-    E.gCr = shfl_dn1(cur.gCr); E.gNb = shfl_dn1(cur.gNb);
-    Point nW, nE;
-    if (NINE) {
-      nW.pd = shfl_up1(nxt.pd); nW.pl = shfl_up1(nxt.pl);
-      nW.gCr = shfl_up1(nxt.gCr); nW.gNb = shfl_up1(nxt.gNb);
-      nE.pd = shfl_dn1(nxt.pd); nE.pl = shfl_dn1(nxt.pl);
-      nE.gCr = shfl_dn1(nxt.gCr); nE.gNb = shfl_dn1(nxt.gNb);
-    }
-
-    // Laplacians accumulated in the reference's mask order
-    // (src/discretization.cu:589-595): rows y-1, y, y+1; columns x-1, x, x+1.
-    double lap[4];
-    {
-      const double sv[4] = {prv.pd, prv.pl, prv.gCr, prv.gNb};
-      const double wv[4] = {W.pd, W.pl, W.gCr, W.gNb};
-      const double cv[4] = {cur.pd, cur.pl, cur.gCr, cur.gNb};
-      const double ev[4] = {E.pd, E.pl, E.gCr, E.gNb};
-      const double nv[4] = {nxt.pd, nxt.pl, nxt.gCr, nxt.gNb};
+  for (int yb = y0;; yb += UNROLL) {
 #pragma unroll
-      for (int f = 0; f < 4; f++) {
-        double v;
-        if (NINE) {
-          const double swv[4] = {prvW.pd, prvW.pl, prvW.gCr, prvW.gNb};
-          const double sev[4] = {prvE.pd, prvE.pl, prvE.gCr, prvE.gNb};
-          const double nwv[4] = {nW.pd, nW.pl, nW.gCr, nW.gNb};
-          const double nev[4] = {nE.pd, nE.pl, nE.gCr, nE.gNb};
-          v = m[0] * swv[f];
-          v = fma(m[1], sv[f], v);
-          v = fma(m[2], sev[f], v);
-          v = fma(m[3], wv[f], v);
-          v = fma(m[4], cv[f], v);
-          v = fma(m[5], ev[f], v);
-          v = fma(m[6], nwv[f], v);
-          v = fma(m[7], nv[f], v);
-          v = fma(m[8], nev[f], v);
-        } else {
-          v = m[1] * sv[f];
-          v = fma(m[3], wv[f], v);
-          v = fma(m[4], cv[f], v);
-          v = fma(m[5], ev[f], v);
-          v = fma(m[7], nv[f], v);
-        }
-        lap[f] = v;
+    for (int k = 0; k < UNROLL; k++) {
+      const int y = yb + k;
+      if (y >= y1) return;
+      const int ps = (k + 2) % 3, cs = k % 3, ns = (k + 1) % 3;
+      const int rs = k % DEPTH;
+      derive<T>(ring[rs], win[ns], cen[ns]);
+      load(ring[rs], min(y + 1 + DEPTH, y1));
+
+      Point W, E;
+      if (NINE) {
+        winW[ns] = shift_left(win[ns]);
+        winE[ns] = shift_right(win[ns]);
+        W = winW[cs];
+        E = winE[cs];
+      } else {
+        W = shift_left(win[cs]);
+        E = shift_right(win[cs]);
       }
-    }
+      double lap[4];
+      laplacians<NINE>(m, win[ps], W, win[cs], E, win[ns], winW[ps], winE[ps], winW[ns],
+                       winE[ns], lap);
 
-    // Cahn-Hilliard (src/discretization.cu:859-863)
-    double nCr = ccur.xCr + P.dt * (P.DCr0 * lap[2] + P.DCr1 * lap[3]);
-    double nNb = ccur.xNb + P.dt * (P.DNb0 * lap[2] + P.DNb1 * lap[3]);
+      const Point& cur = win[cs];
+      const Centre& cc = cen[cs];
+      // Cahn-Hilliard (src/discretization.cu:859-863)
+      double nCr = cc.xCr + P.dt * (P.DCr0 * lap[2] + P.DCr1 * lap[3]);
+      double nNb = cc.xNb + P.dt * (P.DNb0 * lap[2] + P.DNb1 * lap[3]);
+      // Allen-Cahn (src/discretization.cu:888-942) with the closed-form
+      // grand-potential difference at the solved potential
+      const double fd = cur.pd, fl = cur.pl;
+      const double dOd = dOmega_del(cc.mu0, cc.mu1);
+      const double dOl = dOmega_lav(cc.mu0, cc.mu1);
+      const double dFd = -pPrime(fd) * dOd + 2.0 * P.w0 * fd * (fd - 1.0) * (2.0 * fd - 1.0) +
+                         2.0 * P.alpha * fd * fl * fl - P.k0 * lap[0];
+      const double dFl = -pPrime(fl) * dOl + 2.0 * P.w1 * fl * (fl - 1.0) * (2.0 * fl - 1.0) +
+                         2.0 * P.alpha * fl * fd * fd - P.k1 * lap[1];
+      double npd = fd - P.dt * P.L0 * dFd;
+      double npl = fl - P.dt * P.L1 * dFl;
 
-    // Allen-Cahn (src/discretization.cu:888-942) with the closed-form
-    // grand-potential difference at the solved potential
-    const double fd = cur.pd, fl = cur.pl;
-    const double dOd = dOmega_del(ccur.mu0, ccur.mu1);
-    const double dOl = dOmega_lav(ccur.mu0, ccur.mu1);
-    const double dFd = -pPrime(fd) * dOd + 2.0 * P.w0 * fd * (fd - 1.0) * (2.0 * fd - 1.0) +
-                       2.0 * P.alpha * fd * fl * fl - P.k0 * lap[0];
-    const double dFl = -pPrime(fl) * dOl + 2.0 * P.w1 * fl * (fl - 1.0) * (2.0 * fl - 1.0) +
-                       2.0 * P.alpha * fl * fd * fd - P.k1 * lap[1];
-    double npd = fd - P.dt * P.L0 * dFd;
-    double npl = fl - P.dt * P.L1 * dFl;
+      const int py = 1 + y;
+      if (NUC && out_lane)
+        nucleate_cell(N, (uint32_t)(py * pitch + 1 + gx), nCr, nNb, npd, npl, S);
 
-    const int py = 1 + y;
-    if (NUC && out_lane) {
-      nucleate_cell(N, (uint32_t)(py * pitch + 1 + gx), nCr, nNb, npd, npl, S);
-    }
-
-    if (out_lane) {
-      const int qx = 1 + gx;
-      const size_t o = (size_t)py * pitch + qx;
-      oCr[o] = (T)nCr;
-      oNb[o] = (T)nNb;
-      opd[o] = (T)npd;
-      opl[o] = (T)npl;
-      // ghost images of edge cells
-      const bool ex = (gx == 0) || (gx == Nx - 1);
-      const bool ey = (y == 0) || (y == Ny - 1);
-      if (ex || ey) {
-        int cx[3], cy[3], ncx = 1, ncy = 1;
-        cx[0] = qx;
-        cy[0] = py;
-        if (BC == 0) {
-          if (gx == 0) cx[ncx++] = 0;
-          if (gx == Nx - 1) cx[ncx++] = Nx + 1;
-          if (y == 0) cy[ncy++] = 0;
-          if (y == Ny - 1) cy[ncy++] = Ny + 1;
-        } else {
-          if (gx == Nx - 1) cx[ncx++] = 0;
-          if (gx == 0) cx[ncx++] = Nx + 1;
-          if (y == Ny - 1) cy[ncy++] = 0;
-          if (y == 0) cy[ncy++] = Ny + 1;
-        }
-        for (int j = 0; j < ncy; j++)
-          for (int i = 0; i < ncx; i++) {
-            if (i == 0 && j == 0) continue;
-            const size_t g2 = (size_t)cy[j] * pitch + cx[i];
-            oCr[g2] = (T)nCr;
-            oNb[g2] = (T)nNb;
-            opd[g2] = (T)npd;
-            opl[g2] = (T)npl;
+      if (out_lane) {
+        const int qx = 1 + gx;
+        store4(oCr, oNb, opd, opl, (size_t)py * pitch + qx, nCr, nNb, npd, npl);
+        // ghost images of edge cells
+        const bool ex = (gx == 0) || (gx == Nx - 1);
+        const bool ey = (y == 0) || (y == Ny - 1);
+        if (ex || ey) {
+          int cx[3], cy[3], ncx = 1, ncy = 1;
+          cx[0] = qx;
+          cy[0] = py;
+          if (BC == 0) {
+            if (gx == 0) cx[ncx++] = 0;
+            if (gx == Nx - 1) cx[ncx++] = Nx + 1;
+            if (y == 0) cy[ncy++] = 0;
+            if (y == Ny - 1) cy[ncy++] = Ny + 1;
+          } else {
+            if (gx == Nx - 1) cx[ncx++] = 0;
+            if (gx == 0) cx[ncx++] = Nx + 1;
+            if (y == Ny - 1) cy[ncy++] = 0;
+            if (y == 0) cy[ncy++] = Ny + 1;
           }
+          for (int j = 0; j < ncy; j++)
+            for (int i = 0; i < ncx; i++) {
+              if (i == 0 && j == 0) continue;
+              store4(oCr, oNb, opd, opl, (size_t)cy[j] * pitch + cx[i], nCr, nNb, npd, npl);
+            }
+        }
       }
     }
-
-    if (NINE) {
-      prvW = W;
-      prvE = E;
-    }
-    prv = cur;
-    cur = nxt;
-    ccur = cnxt;
   }
 }
 
@@ -304,12 +336,12 @@ int default_rows(int nx, int ny) {
   // amortise the two halo rows
   const int Nx = nx - 2, Ny = ny - 2;
   const int strips = (Nx + STRIP - 1) / STRIP;
-  int rows = 64;
-  while (rows > 8 && (long)strips * ((Ny + rows - 1) / rows) < 4096) rows >>= 1;
+  int rows = 16;
+  while (rows > 4 && (long)strips * ((Ny + rows - 1) / rows) < 16384) rows >>= 1;
   return rows;
 }
 
-template <typename T, int BC, bool NINE, bool NUC>
+template <typename T, int BC, bool NINE, bool NUC, int DEPTH>
 static hipError_t launch_step_t(int nx, int ny, const StepParams& P, const T* iCr, const T* iNb,
                                 const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd, T* opl,
                                 const NucParams& N, const Scratch& S, const TileConfig& tc,
@@ -318,10 +350,28 @@ static hipError_t launch_step_t(int nx, int ny, const StepParams& P, const T* iC
   const int rows = tc.rows > 0 ? tc.rows : default_rows(nx, ny);
   const int strips = (Nx + STRIP - 1) / STRIP;
   const int chunks = (Ny + rows - 1) / rows;
-  hipLaunchKernelGGL((step_kernel<T, BC, NINE, NUC>), dim3(strips * chunks), dim3(WAVE), 0, st,
-                     iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny, nx, strips, chunks, rows,
-                     tc.remap, P, N, S);
+  const int blocks = (strips * chunks + WPB - 1) / WPB;
+  hipLaunchKernelGGL((step_kernel<T, BC, NINE, NUC, DEPTH>), dim3(blocks), dim3(WAVE * WPB),
+                     0, st, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny, nx, strips, chunks,
+                     rows, tc.remap, P, N, S);
   return hipGetLastError();
+}
+
+// The plain five-point step without nucleation (every step but one in ~900)
+// is instantiated for prefetch depths 2, 3 and 6; the rarer variants use 3.
+template <typename T, int BC>
+static hipError_t launch_plain(int nx, int ny, const StepParams& P, const T* iCr, const T* iNb,
+                               const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd, T* opl,
+                               const NucParams& N, const Scratch& S, const TileConfig& tc,
+                               hipStream_t st) {
+  if (tc.depth == 2)
+    return launch_step_t<T, BC, false, false, 2>(nx, ny, P, iCr, iNb, ipd, ipl, oCr, oNb, opd,
+                                                 opl, N, S, tc, st);
+  if (tc.depth == 6)
+    return launch_step_t<T, BC, false, false, 6>(nx, ny, P, iCr, iNb, ipd, ipl, oCr, oNb, opd,
+                                                 opl, N, S, tc, st);
+  return launch_step_t<T, BC, false, false, 3>(nx, ny, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl,
+                                               N, S, tc, st);
 }
 
 template <typename T>
@@ -332,23 +382,23 @@ hipError_t launch_step(int nx, int ny, int bc, const StepParams& P, const T* iCr
   const bool nine = P.m[0] != 0.0 || P.m[2] != 0.0 || P.m[6] != 0.0 || P.m[8] != 0.0;
   NucParams N{};
   if (nuc) N = *nuc;
-#define PF_LAUNCH(BCV, NI, NU)                                                               \
-  return launch_step_t<T, BCV, NI, NU>(nx, ny, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, N, \
-                                       S, tc, st)
+#define PF_LAUNCH(BCV, NI, NU)                                                                  \
+  return launch_step_t<T, BCV, NI, NU, 3>(nx, ny, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, N, \
+                                          S, tc, st)
   if (bc == 0) {
     if (nine) {
       if (nuc) PF_LAUNCH(0, true, true);
       PF_LAUNCH(0, true, false);
     }
     if (nuc) PF_LAUNCH(0, false, true);
-    PF_LAUNCH(0, false, false);
+    return launch_plain<T, 0>(nx, ny, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, N, S, tc, st);
   } else {
     if (nine) {
       if (nuc) PF_LAUNCH(1, true, true);
       PF_LAUNCH(1, true, false);
     }
     if (nuc) PF_LAUNCH(1, false, true);
-    PF_LAUNCH(1, false, false);
+    return launch_plain<T, 1>(nx, ny, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, N, S, tc, st);
   }
 #undef PF_LAUNCH
 }

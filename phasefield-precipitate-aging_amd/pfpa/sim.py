@@ -66,8 +66,8 @@ class Simulation:
         self._step = lib().pf_step_f64 if dtype == torch.float64 else lib().pf_step_f32
         self._summ = lib().pf_summarize_f64 if dtype == torch.float64 else lib().pf_summarize_f32
 
-    def set_tile(self, rows, remap=1):
-        check(lib().pf_ctx_set_tile(self.ctx, rows, remap), "pf_ctx_set_tile")
+    def set_tile(self, rows, remap=1, depth=0):
+        check(lib().pf_ctx_set_tile(self.ctx, rows, remap, depth), "pf_ctx_set_tile")
 
     def step(self, n=1):
         st = ctypes.c_void_p(self.stream.cuda_stream)

@@ -174,16 +174,17 @@ def test_ragged_shapes(gpu, oracle, shape, bc):
     f0 = [M.fill_ghosts(a, bc) for a in (xCr, xNb, pd, pl)]
     want = oracle.run(f0, oracle_params(oracle, s), bc, 5)
     results = []
-    for rows, remap in ((0, 1), (1, 0), (3, 1), (16, 1), (64, 0)):
+    for rows, remap, depth in ((0, 1, 0), (1, 0, 2), (3, 1, 6), (16, 1, 2), (64, 0, 3),
+                               (7, 1, 6)):
         sim = Simulation(s, f0)
-        sim.set_tile(rows, remap)
+        sim.set_tile(rows, remap, depth)
         sim.step(5)
         got = sim.fields()
         for k in range(4):
-            assert mixed_err(got[k], want[k]) <= TOL, (rows, remap, k)
+            assert mixed_err(got[k], want[k]) <= TOL, (rows, remap, depth, k)
             # ghost ring holds the boundary image of the interior
             g = M.fill_ghosts(got[k].copy(), bc)
-            assert np.array_equal(g, got[k]), (rows, remap, k)
+            assert np.array_equal(g, got[k]), (rows, remap, depth, k)
         results.append(got)
         sim.close()
     # tile shape does not change a single bit
