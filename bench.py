@@ -1,22 +1,34 @@
 """Benchmark of the KKS phase-field step on one MI355X.
 
-Default workload (the plain form): 4096 x 4096 interior cells, FP64,
+Headline (value): the plain form on 4096 x 4096 interior cells, FP64,
 dx = 0.25 nm, periodic, seeded uniform matrix composition in
-Ni-(0.0202..0.0269)Nb-(0.2794..0.3288)Cr with a central enriched strip
-(1/8 of the width) at Ni-0.169Nb-0.272Cr, nucleation on (Philox seed 42,
-every floor(1e-4/dt) steps, fused into the step), dt = 0.1 dx^2 / (4 D_max).
+Ni-(0.0202..0.0269)Nb-(0.2794..0.3288)Cr with a central enriched strip (1/8 of
+the width) at Ni-0.169Nb-0.272Cr, nucleation on (Philox seed 42, every
+floor(1e-4/dt) steps, fused into the step), dt = 0.1 dx^2 / (4 D_max).
+A "step" is one fused launch over the whole grid.  value = interior cells x
+timed steps / time of the timed loop (HIP events), fields resident in HBM.
+roofline: algorithmic bytes of the step kernel (64 B per cell-update in FP64:
+4 fields read + 4 written) over its average launch duration (HIP events on
+the launch stream); traffic: HBM bytes per launch from the rocprofv3
+FETCH_SIZE / WRITE_SIZE runs under profiles/ (scaled by a copy kernel).
+cpu_baseline: the FP64 CPU oracle (OpenMP, all host threads) on a bounded
+sample of the same workload.
 
-A "step" is one fused launch over the whole grid (the nucleation pass rides
-along on its steps).  value = interior cells x timed steps / wall time of the
-timed loop, fields resident in HBM.  roofline: algorithmic bytes of the step
-kernel (64 B per cell-update in FP64: 4 fields read + 4 written) over its
-average launch duration measured with HIP events on the launch stream.
-cpu_baseline: the FP64 CPU oracle (OpenMP) on this host, on a bounded sample
-of the same workload.
+Also reported (same JSON line):
+  batched  -- 512 independent simulations, square sides log-uniform in
+              128..1024, FP32 storage with FP64 arithmetic and reductions, one
+              persistent launch per step replayed from a hipGraph, tile and
+              steps-per-graph from the autotuner; against a loop of plain FP64
+              launches over the same instances; phase-fraction error against
+              FP64 replays of sampled instances.
+  serving  -- Poisson sweep requests (side log-uniform 128..2048, 500..20000
+              steps, random composition and nucleation seed) with continuous
+              batching under an HBM budget; throughput and p50/p99 latency.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--form plain|precision]
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -28,6 +40,11 @@ for p in (os.path.join(ROOT, "phasefield-precipitate-aging_amd"), os.path.join(R
         sys.path.insert(0, p)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+METRIC = "cell-updates/s (% HBM roofline) plain vs batched; p50/p99 sweep latency"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
 def load_traffic(workload):
@@ -42,7 +59,7 @@ def load_traffic(workload):
 
 
 def cpu_baseline(N, steps, warmup=1):
-    """Time the FP64 oracle (all host cores via OpenMP) on N x N for `steps`."""
+    """Time the FP64 oracle (all host threads via OpenMP) on N x N for `steps`."""
     import oracle as O
     from pfpa import model as M
     s = M.make_setup(N, N, bc=1)
@@ -55,19 +72,263 @@ def cpu_baseline(N, steps, warmup=1):
     return N * N * steps / el, O.num_threads(), el
 
 
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _params(s):
+    from pfpa._lib import PfParams
+    P = PfParams()
+    P.dt = s.dt
+    for k in range(2):
+        P.DCr[k], P.DNb[k] = s.DCr[k], s.DNb[k]
+        P.kappa[k], P.omega[k], P.Lmob[k] = s.kappa[k], s.omega[k], s.Lmob[k]
+    P.alpha = s.alpha
+    for k in range(9):
+        P.mask[k] = s.mask[k]
+    return P
+
+
+def plain_section(args, dtype_name="f64"):
+    import torch
+
+    from pfpa import model as M
+    from pfpa._lib import check, lib
+    from pfpa.sim import Simulation
+
+    N = args.size
+    dtype = torch.float64 if dtype_name == "f64" else torch.float32
+    s = M.make_setup(N, N, bc=1)
+    f0 = M.init_enriched_band(N, N, 1, seed=42)
+    sim = Simulation(s, f0, dtype=dtype, nucleation=True, seed=42)
+    stream = sim.stream
+    sim.step(args.warmup)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    sim.step(args.steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    cells = N * N
+    value = cells * args.steps / (ms / 1e3)
+
+    # the dominant kernel alone: back-to-back step launches (no nucleation steps)
+    P = sim.P
+    a, b = sim.a, sim.b
+    st = ctypes.c_void_p(stream.cuda_stream)
+    fn = lib().pf_step_f64 if dtype == torch.float64 else lib().pf_step_f32
+    reps = max(20, min(200, args.steps))
+    k0 = torch.cuda.Event(enable_timing=True)
+    k1 = torch.cuda.Event(enable_timing=True)
+    k0.record(stream)
+    for i in range(reps):
+        x, y = (a, b) if i % 2 == 0 else (b, a)
+        check(fn(sim.ctx, s.nx, s.ny, s.bc, ctypes.byref(P), *[_ptr(t) for t in x],
+                 *[_ptr(t) for t in y], None, st), "step")
+    k1.record(stream)
+    torch.cuda.synchronize()
+    kern_s = k0.elapsed_time(k1) / 1e3 / reps
+    bpc = 64 if dtype == torch.float64 else 32
+    alg_bytes = bpc * cells
+    achieved = alg_bytes / kern_s
+    workload = ("plain_fp64_%d" if dtype == torch.float64 else "precision_fp32_%d") % N
+    traffic = load_traffic(workload)
+    nuc_every = sim.nuc_every
+    sim.close()
+    return {
+        "value": value, "ms_per_step": ms / args.steps, "N": N, "nuc_every": nuc_every,
+        "dt": s.dt,
+        "roofline": {
+            "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": traffic,
+            "kernel": "pf::step_kernel (fused step)", "kernel_avg_us": kern_s * 1e6,
+            "algorithmic_bytes_per_launch": alg_bytes,
+        },
+    }
+
+
+def batched_section(args):
+    """512 uneven simulations: batched FP32 vs a loop of plain FP64 launches."""
+    import torch
+
+    from pfpa._lib import PfStepItem, check, lib
+    from pfpa.batch import PF_STORAGE_F32, Batch, job_bytes, make_desc
+    from pfpa.sim import Simulation
+    from pfpa.workloads import autotune_batch, batch_specs, spec_fields, spec_setup
+
+    specs = batch_specs(args.batch_instances, seed=7)
+    cells = sum(sp["n"] ** 2 for sp in specs)
+    tune_steps = 8
+    pool = sum(job_bytes(PF_STORAGE_F32, sp["n"], sp["n"]) for sp in specs) + (64 << 20)
+    batch = Batch(PF_STORAGE_F32, pool_bytes=pool, max_jobs=len(specs))
+    total_steps = args.batch_steps
+    target = total_steps + 40 * tune_steps
+    plain_inputs = []
+    t_gen = time.perf_counter()
+    for sp in specs:
+        s = spec_setup(sp)
+        f = spec_fields(sp)
+        batch.add(make_desc(s, target, nucleation=True, seed=1000 + sp["index"],
+                            instance=sp["index"]), f)
+        plain_inputs.append((s, f))
+    log("batched: %d instances, %.3g cells, setup %.1f s" % (len(specs), cells,
+                                                            time.perf_counter() - t_gen))
+    best, table = autotune_batch(batch, steps=tune_steps)
+    tuned = {"rows": best[1], "depth": best[2], "steps_per_graph": best[3],
+             "candidates": len(table)}
+    spg = best[3]
+    batch.run(spg, spg)
+    batch.sync()
+    steps_before = batch.poll(0)[0]
+    t0 = time.perf_counter()
+    batch.run(total_steps, spg)
+    batch.sync()
+    el = time.perf_counter() - t0
+    batched_rate = cells * total_steps / el
+    steps_after = batch.poll(0)[0]
+    assert steps_after - steps_before == total_steps, (steps_before, steps_after)
+    # phase fractions of sampled jobs after steps_after steps
+    order = sorted(range(len(specs)), key=lambda k: specs[k]["n"])
+    sample = [order[i] for i in (0, len(order) // 3, 2 * len(order) // 3)]
+    batched_summ = {k: batch.summarize(k) for k in sample}
+    batch.status()
+    batch.close()
+
+    # loop of plain FP64 launches (one per instance per step), issued from C
+    ctx = ctypes.c_void_p()
+    check(lib().pf_ctx_create(ctypes.byref(ctx)), "ctx")
+    bufs = []
+    prms = []
+    for s, f in plain_inputs:
+        A = [torch.as_tensor(x).cuda() for x in f]
+        B = [torch.empty_like(x) for x in A]
+        bufs.append((A, B))
+        prms.append(_params(s))
+    items = [(PfStepItem * len(bufs))(), (PfStepItem * len(bufs))()]
+    for par in range(2):
+        for i, ((A, B), (s, _), P) in enumerate(zip(bufs, plain_inputs, prms)):
+            it = items[par][i]
+            it.nx, it.ny, it.bc = s.nx, s.ny, s.bc
+            it.prm = ctypes.pointer(P)
+            src, dst = (A, B) if par == 0 else (B, A)
+            for k in range(4):
+                it.old_fields[k] = src[k].data_ptr()
+                it.new_fields[k] = dst[k].data_ptr()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for par in range(2):
+        check(lib().pf_step_loop_f64(ctx, len(bufs), items[par], st), "loop")
+    torch.cuda.synchronize()
+    loop_steps = args.loop_steps
+    t0 = time.perf_counter()
+    for i in range(loop_steps):
+        check(lib().pf_step_loop_f64(ctx, len(bufs), items[i % 2], st), "loop")
+    torch.cuda.synchronize()
+    loop_el = time.perf_counter() - t0
+    loop_rate = cells * loop_steps / loop_el
+    lib().pf_ctx_destroy(ctx)
+    del bufs
+
+    # FP64 replays of the sampled jobs for the phase-fraction error
+    errs = []
+    for k in sample:
+        s, f = plain_inputs[k]
+        sim = Simulation(s, f, dtype=torch.float64, nucleation=True, seed=1000 + k, instance=k)
+        sim.step(steps_after)
+        ref = sim.summarize()
+        sim.close()
+        errs.append(max(abs(ref[3] - batched_summ[k][3]), abs(ref[4] - batched_summ[k][4])))
+    return {
+        "instances": len(specs), "cells_per_step": cells, "steps": total_steps,
+        "dtype": "f32-storage/f64-compute",
+        "value": batched_rate, "unit": "cell-updates/s",
+        "roofline_frac_f32": batched_rate * 32 / HBM_PEAK,
+        "plain_loop_fp64_value": loop_rate, "plain_loop_steps": loop_steps,
+        "speedup_vs_plain_loop": batched_rate / loop_rate,
+        "autotuned": tuned,
+        "phase_fraction_max_abs_error_vs_fp64": max(errs),
+        "phase_fraction_sample": {"jobs": sample, "steps": steps_after},
+    }
+
+
+def serving_section(args):
+    from pfpa.serve import fp64_replay_error, make_requests, serve
+    reqs = make_requests(args.serve_rate, args.serve_duration, seed=1)
+    finished, stats = serve(reqs, pool_bytes=args.serve_pool_gb << 30,
+                            steps_per_graph=args.serve_spg, rows=8, depth=3)
+    sample = sorted(finished, key=lambda r: r.n * r.n * r.steps)[:3]
+    err, _ = fp64_replay_error(finished, sample)
+    stats["arrival_rate_per_s"] = args.serve_rate
+    stats["arrival_window_s"] = args.serve_duration
+    stats["phase_fraction_max_abs_error_vs_fp64"] = err
+    stats["fp64_replayed"] = len(sample)
+    return stats
+
+
+def precision_section(args):
+    """FP32 storage vs FP64 on 8192^2 (same init as the plain case): max field
+    differences, phase fractions and mass drift at 1k/10k/20k steps."""
+    import torch
+
+    from pfpa import model as M
+    from pfpa.sim import Simulation
+    N = args.size
+    s = M.make_setup(N, N, bc=1)
+    f0 = M.init_enriched_band(N, N, 1, seed=42)
+    s64 = Simulation(s, f0, dtype=torch.float64, nucleation=True, seed=42)
+    s32 = Simulation(s, f0, dtype=torch.float32, nucleation=True, seed=42)
+    del f0
+    m0 = s64.summarize()
+    rows = []
+    done = 0
+    for mark in args.precision_marks:
+        s64.step(mark - done)
+        s32.step(mark - done)
+        done = mark
+        dCr = float((s64.a[0] - s32.a[0].double()).abs().max())
+        dNb = float((s64.a[1] - s32.a[1].double()).abs().max())
+        r64 = s64.summarize()
+        r32 = s32.summarize()
+        rows.append({"step": mark, "max_abs_dx_Cr": dCr, "max_abs_dx_Nb": dNb,
+                     "delta_frac_64": r64[3], "delta_frac_32": r32[3],
+                     "laves_frac_64": r64[4], "laves_frac_32": r32[4],
+                     "mass_drift_Cr_64": r64[0] - m0[0], "mass_drift_Cr_32": r32[0] - m0[0],
+                     "mass_drift_Nb_64": r64[1] - m0[1], "mass_drift_Nb_32": r32[1] - m0[1]})
+        log("precision", json.dumps(rows[-1]))
+    s64.close()
+    s32.close()
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--form", choices=["plain", "precision"], default="plain")
-    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-only", action="store_true")
+    ap.add_argument("--batch-instances", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=5000)
+    ap.add_argument("--loop-steps", type=int, default=20)
+    ap.add_argument("--serve-rate", type=float, default=6.0)
+    ap.add_argument("--serve-duration", type=float, default=20.0)
+    ap.add_argument("--serve-pool-gb", type=int, default=4)
+    ap.add_argument("--serve-spg", type=int, default=50)
+    ap.add_argument("--precision-marks", type=int, nargs="+", default=[1000, 10000, 20000])
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain device copy of 4 x 4098^2 FP64 arrays (1.07 GB moved per "
                          "dispatch) to scale rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(multi-GPU sharding is described in DESIGN.md, not built)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
 
     if args.calibrate_copy:
         import torch
@@ -78,115 +339,58 @@ def main():
             for a, b in zip(dst, src):
                 a.copy_(b)
         torch.cuda.synchronize()
-        print(json.dumps({"calibration": "torch copy_ of 4 x %d FP64", "bytes_read_per_dispatch":
-                          n * 8, "bytes_written_per_dispatch": n * 8}))
+        print(json.dumps({"calibration": "torch copy_ of 4 x %d FP64" % n,
+                          "bytes_read_per_dispatch": n * 8, "bytes_written_per_dispatch": n * 8}))
         return
 
-    if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
-              "(multi-GPU sharding is described in DESIGN.md, not built)" % args.gpus,
-              file=sys.stderr)
-        sys.exit(2)
+    if args.form == "precision":
+        args.size = args.size or 8192
+        rows = precision_section(args)
+        print(json.dumps({"form": "precision", "grid": [args.size, args.size], "rows": rows}))
+        return
 
-    import ctypes
-
-    import torch
-
-    from pfpa import model as M
-    from pfpa._lib import PfParams, check, lib
-    from pfpa.sim import Simulation
-
-    N = args.size
-    dtype = torch.float64 if args.form == "plain" else torch.float32
-    s = M.make_setup(N, N, bc=1)
-    f0 = M.init_enriched_band(N, N, 1, seed=42)
-    sim = Simulation(s, f0, dtype=dtype, nucleation=True, seed=42)
-    stream = sim.stream
-
-    sim.step(args.warmup)
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record(stream)
-    sim.step(args.steps)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = e0.elapsed_time(e1)
-    cells = N * N
-    value = cells * args.steps / (ms / 1e3)
-
-    # dominant kernel alone: back-to-back step launches (no nucleation steps)
-    P = sim.P
-    a, b = sim.a, sim.b
-    pt = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
-    st = ctypes.c_void_p(stream.cuda_stream)
-    fn = lib().pf_step_f64 if dtype == torch.float64 else lib().pf_step_f32
-    reps = max(20, min(200, args.steps))
-    k0 = torch.cuda.Event(enable_timing=True)
-    k1 = torch.cuda.Event(enable_timing=True)
-    k0.record(stream)
-    for i in range(reps):
-        x, y = (a, b) if i % 2 == 0 else (b, a)
-        check(fn(sim.ctx, s.nx, s.ny, s.bc, ctypes.byref(P), *[pt(t) for t in x],
-                 *[pt(t) for t in y], None, st), "step")
-    k1.record(stream)
-    torch.cuda.synchronize()
-    kern_s = k0.elapsed_time(k1) / 1e3 / reps
-    bpc = 64 if dtype == torch.float64 else 32
-    alg_bytes = bpc * cells
-    achieved = alg_bytes / kern_s
-    workload = "plain_fp64_%d" % N if args.form == "plain" else "precision_fp32_%d" % N
-    traffic = load_traffic(workload)
-
-    mass = sim.summarize()
-    sim.close()
-
-    cpu = None
-    if not args.no_cpu:
-        cpu_N = 4096 if N >= 4096 else N
-        v, cores, el = cpu_baseline(cpu_N, args.cpu_steps)
-        cpu = {"value": v, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-               "sample": "%d steps of the %dx%d FP64 plain step (same init, periodic) after "
-                         "1 warm-up step, %.1f s" % (args.cpu_steps, cpu_N, cpu_N, el)}
-
+    args.size = args.size or 4096
+    t_all = time.perf_counter()
+    plain = plain_section(args)
+    log("plain: %.4g cell-updates/s, %.1f us/step kernel" %
+        (plain["value"], plain["roofline"]["kernel_avg_us"]))
     out = {
-        "metric": "cell-updates/s (% HBM roofline) plain vs batched; p50/p99 sweep latency",
-        "value": value,
+        "metric": METRIC,
+        "value": plain["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms / args.steps,
+        "ms_per_step": plain["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if dtype == torch.float64 else "f32-storage/f64-compute",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "%s form, %dx%d interior, dx=0.25nm, periodic, nucleation on "
+            "workload": "plain form, %dx%d interior, dx=0.25nm, periodic, nucleation on "
                         "(Philox seed 42, every %d steps, fused), dt=%.4g s"
-                        % (args.form, N, N, sim.nuc_every, s.dt),
-            "grid": [N, N],
-            "form": args.form,
+                        % (args.size, args.size, plain["nuc_every"], plain["dt"]),
+            "grid": [args.size, args.size],
+            "form": "plain",
             "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved / 1e9,
-            "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK,
-            "traffic": traffic,
-            "kernel": "pf::step_kernel (fused step)",
-            "kernel_avg_us": kern_s * 1e6,
-            "algorithmic_bytes_per_launch": alg_bytes,
-        },
-        "cpu_baseline": cpu,
-        "wall_s": wall,
-        "mean_fields": mass,
+        "roofline": plain["roofline"],
     }
+    if not args.plain_only:
+        out["batched"] = batched_section(args)
+        log("batched:", json.dumps(out["batched"]))
+        out["serving"] = serving_section(args)
+        log("serving:", json.dumps(out["serving"]))
+    cpu = None
+    if not args.no_cpu:
+        cpu_N = 4096 if args.size >= 4096 else args.size
+        v, cores, el = cpu_baseline(cpu_N, args.cpu_steps)
+        cpu = {"value": v, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+               "sample": "%d steps of the %dx%d FP64 plain step (same init, periodic) after "
+                         "1 warm-up step, %.1f s" % (args.cpu_steps, cpu_N, cpu_N, el)}
+    out["cpu_baseline"] = cpu
+    out["bench_wall_s"] = time.perf_counter() - t_all
     print(json.dumps(out))
 
 

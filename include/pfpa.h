@@ -131,8 +131,77 @@ int pf_thermo_eval(int what, int n, const double* in, double* out, void* stream)
  * init_prng_kernel (src/discretization.cu:1022-1044). */
 int pf_philox_eval(int n, const uint32_t* in, uint32_t* out, void* stream);
 
+/* A loop of plain FP64 steps over independent grids, one launch each, issued
+ * from C: the "loop of plain launches" the batched form is compared with. */
+typedef struct pf_step_item {
+	int nx, ny, bc;
+	const pf_params* prm;
+	const double* old_fields[4];   /* conc_Cr, conc_Nb, phi_del, phi_lav */
+	double* new_fields[4];
+} pf_step_item;
+int pf_step_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream);
+
 /* Heuristic rows-per-block the step kernel uses for an nx x ny grid. */
 int pf_default_rows(int nx, int ny);
+
+/* ------------------------------------------------------------------------
+ * Batched form: many independent simulations of uneven size advanced by one
+ * persistent launch per step (no counterpart in the reference, which runs one
+ * grid per process).  Jobs live in one device pool; their fields stay
+ * resident between calls.  A job's result does not depend on the other jobs
+ * in the batch.
+ * ------------------------------------------------------------------------ */
+#define PF_STORAGE_F64 0
+#define PF_STORAGE_F32 1   /* fields in FP32, arithmetic and reductions in FP64 */
+
+typedef struct pf_job_desc {
+	int Nx, Ny;            /* interior size; arrays are (Nx+2) x (Ny+2) with the ghost ring */
+	int bc;                /* PF_BC_NOFLUX or PF_BC_PERIODIC */
+	int nucleation;        /* 1 = nucleation pass every nuc_every steps (step % nuc_every == 0) */
+	double dx;             /* mesh spacing (m), used by the nucleation pass */
+	pf_params prm;         /* dt, diffusivity, kappa, omega, mobility, alpha, five-point mask */
+	uint64_t steps;        /* steps to run */
+	uint64_t seed;         /* Philox key */
+	uint32_t instance;     /* folded into the random counter: unique per simulation */
+	int nuc_every;
+	double sigma_del, sigma_lav, unit_a;
+} pf_job_desc;
+
+typedef struct pf_batch pf_batch;
+
+/* HBM a job needs (two buffer sets of four planes). */
+size_t pf_batch_job_bytes(int storage, int Nx, int Ny);
+int pf_batch_create(pf_batch** batch, int storage, size_t pool_bytes, int max_jobs);
+void pf_batch_destroy(pf_batch* batch);
+/* Largest contiguous free block of the pool, in bytes. */
+size_t pf_batch_free_bytes(pf_batch* batch);
+/* Rows per strip task and prefetch depth (2, 3 or 6) of the batched step. */
+int pf_batch_set_tile(pf_batch* batch, int rows, int depth);
+/* Admit a job; fields are HOST arrays of the batch's storage type, (Nx+2)*(Ny+2) each.
+ * Returns PF_ERR_OVERFLOW when the pool has no room (the caller may retry later). */
+int pf_batch_add(pf_batch* batch, const pf_job_desc* desc, const void* conc_Cr,
+                 const void* conc_Nb, const void* phi_del, const void* phi_lav, int* job_id);
+/* Retire a job and return its memory to the pool. */
+int pf_batch_remove(pf_batch* batch, int job_id);
+/* Publish admissions/removals to the device tables (done by pf_batch_run too). */
+int pf_batch_commit(pf_batch* batch);
+/* Enqueue nsteps steps of every active job; with steps_per_graph > 1 the steps
+ * are replayed from a captured hipGraph of that many steps. Asynchronous. */
+int pf_batch_run(pf_batch* batch, int nsteps, int steps_per_graph);
+int pf_batch_sync(pf_batch* batch);
+/* Synchronous job state. */
+int pf_batch_poll(pf_batch* batch, int job_id, uint64_t* steps_done, int* active);
+int pf_batch_poll_all(pf_batch* batch, int capacity, int* job_ids, uint64_t* steps_done,
+                      int* active, int* count);
+/* Synchronous copy of a job's current fields to host arrays (NULL skips a field). */
+int pf_batch_read(pf_batch* batch, int job_id, void* conc_Cr, void* conc_Nb, void* phi_del,
+                  void* phi_lav);
+/* Synchronous per-job means {x_Cr, x_Nb, gamma, delta, Laves} (compensated FP64). */
+int pf_batch_summarize(pf_batch* batch, int job_id, double* out5);
+/* Fails if any nucleation event needed an embed wider than one cell. */
+int pf_batch_status(pf_batch* batch, uint64_t* wide_events);
+/* Strip tasks per step of the active jobs (for throughput accounting). */
+int pf_batch_waves(pf_batch* batch);
 
 /* Host-side evaluation of the same pointwise thermodynamics (no GPU needed).
  * Replace the host functions of thermo/parabola625.h the driver calls:

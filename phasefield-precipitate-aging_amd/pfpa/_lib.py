@@ -35,6 +35,12 @@ class PfNucParams(ctypes.Structure):
                 ("reserved", ctypes.c_uint32)]
 
 
+class PfStepItem(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("bc", ctypes.c_int),
+                ("prm", ctypes.POINTER(PfParams)), ("old_fields", ctypes.c_void_p * 4),
+                ("new_fields", ctypes.c_void_p * 4)]
+
+
 class PfError(RuntimeError):
     pass
 
@@ -59,6 +65,7 @@ PROTOTYPES = {
     "pf_thermo_eval": (_i, [_i, _i, _vp, _vp, _vp]),
     "pf_philox_eval": (_i, [_i, _vp, _vp, _vp]),
     "pf_default_rows": (_i, [_i, _i]),
+    "pf_step_loop_f64": (_i, [_vp, _i, ctypes.POINTER(PfStepItem), _vp]),
     "pf_host_p": (_d, [_d]),
     "pf_host_pPrime": (_d, [_d]),
     "pf_host_g": (_d, [_i, _d, _d]),

@@ -21,7 +21,7 @@ class Simulation:
     (FP32 storage, FP64 arithmetic)."""
 
     def __init__(self, setup, fields, dtype=torch.float64, nucleation=False, seed=42,
-                 nuc_every=None, instance=0, stream=None, ctx=None):
+                 nuc_every=None, instance=0, stream=None, ctx=None, nuc_dx=None):
         if not torch.cuda.is_available():
             raise RuntimeError("Simulation needs a GPU")
         self.s = setup
@@ -56,7 +56,8 @@ class Simulation:
         N.sigma_del, N.sigma_lav = SIGMA
         N.unit_a = LATTICE_CONST
         N.ifce_width = IFCE_WIDTH
-        N.dx = N.dy = N.dz = setup.dx
+        # mesh spacing the nucleation pass uses (the stencil's by default)
+        N.dx = N.dy = N.dz = nuc_dx if nuc_dx else setup.dx
         N.dt = self.nuc_every * setup.dt
         N.seed = seed
         N.call = 0

@@ -1,0 +1,168 @@
+"""Batched form: every job gives the same bits as the plain path run alone,
+alone or in any mix, eager or replayed from a hipGraph."""
+import numpy as np
+import pytest
+import torch
+
+from pfpa import model as M
+from pfpa.batch import PF_STORAGE_F32, PF_STORAGE_F64, Batch, make_desc
+from pfpa.sim import Simulation
+
+pytestmark = pytest.mark.gpu
+
+NUC_DX = 4.83e-6   # nucleation-pass mesh spacing that puts P inside (0, 1) on 66 x 50 grids
+
+
+def rand_fields(Nx, Ny, bc, seed, particles=True):
+    rng = np.random.default_rng(seed)
+    xCr = 0.3 + 0.01 * rng.standard_normal((Ny + 2, Nx + 2))
+    xNb = 0.02 + 0.005 * rng.standard_normal((Ny + 2, Nx + 2))
+    pd = np.zeros((Ny + 2, Nx + 2))
+    pl = np.zeros((Ny + 2, Nx + 2))
+    if particles:
+        pd[2:Ny // 3, 2:Nx // 3] = 1.0
+        pl[Ny // 2:Ny // 2 + 4, Nx // 2:Nx // 2 + 5] = 0.7
+    return [M.fill_ghosts(a, bc) for a in (xCr, xNb, pd, pl)]
+
+
+def nuc_fields(Nx, Ny, bc, seed):
+    rng = np.random.default_rng(seed)
+    xCr = rng.uniform(0.15, 0.3, (Ny + 2, Nx + 2))
+    xNb = rng.uniform(0.19, 0.24, (Ny + 2, Nx + 2))
+    z = np.zeros((Ny + 2, Nx + 2))
+    return [M.fill_ghosts(a, bc) for a in (xCr, xNb, z, z.copy())]
+
+
+def plain(setup, fields, steps, dtype, **kw):
+    sim = Simulation(setup, fields, dtype=dtype, **kw)
+    sim.step(steps)
+    out = sim.fields()
+    sim.close()
+    return out
+
+
+JOBS = [(64, 64, 1), (130, 71, 0), (37, 200, 1), (200, 33, 0), (1, 5, 1), (63, 62, 0)]
+
+
+@pytest.mark.parametrize("storage", [PF_STORAGE_F64, PF_STORAGE_F32])
+def test_batch_equals_plain_alone(gpu, storage):
+    dtype = torch.float64 if storage == PF_STORAGE_F64 else torch.float32
+    b = Batch(storage, pool_bytes=256 << 20)
+    b.set_tile(8, 3)
+    ids = []
+    setups = []
+    for k, (Nx, Ny, bc) in enumerate(JOBS):
+        s = M.make_setup(Nx, Ny, bc=bc)
+        f = rand_fields(Nx, Ny, bc, 100 + k)
+        ids.append(b.add(make_desc(s, steps=17 + k, instance=k), f))
+        setups.append((s, f, 17 + k))
+    b.run(40)          # more steps than any job's target: each stops at its own
+    for jid, (s, f, n) in zip(ids, setups):
+        steps, active = b.poll(jid)
+        assert steps == n and not active
+        got = b.read(jid)
+        want = plain(s, f, n, dtype)
+        for k in range(4):
+            assert np.array_equal(got[k].astype(np.float64), want[k]), (jid, k)
+    b.close()
+
+
+def test_alone_vs_mix_graph_and_eager(gpu):
+    """FP32 job X: alone (eager) == in a mix (graph replays, admissions in
+    between) == the plain FP32 path."""
+    s = M.make_setup(96, 80, bc=1)
+    fx = rand_fields(96, 80, 1, 7)
+    steps = 30
+    alone = Batch(PF_STORAGE_F32, pool_bytes=64 << 20)
+    jx = alone.add(make_desc(s, steps, instance=5), fx)
+    alone.run(steps)
+    a = alone.read(jx)
+    alone.close()
+
+    mix = Batch(PF_STORAGE_F32, pool_bytes=256 << 20)
+    mix.set_tile(4, 6)
+    others = []
+    for k, (Nx, Ny, bc) in enumerate(JOBS):
+        so = M.make_setup(Nx, Ny, bc=bc)
+        others.append(mix.add(make_desc(so, 1000, instance=50 + k), rand_fields(Nx, Ny, bc, k)))
+    mix.run(7, steps_per_graph=3)          # others advance before X arrives
+    jx2 = mix.add(make_desc(s, steps, instance=5), fx)
+    mix.run(10, steps_per_graph=4)
+    mix.remove(others[1])                  # retire one while X runs
+    so = M.make_setup(150, 90, bc=0)
+    mix.add(make_desc(so, 1000, instance=99), rand_fields(150, 90, 0, 3))
+    mix.run(steps - 10, steps_per_graph=4)
+    assert mix.poll(jx2) == (steps, False)
+    m = mix.read(jx2)
+    mix.close()
+    want = plain(s, fx, steps, torch.float32)
+    for k in range(4):
+        assert np.array_equal(a[k], m[k]), k
+        assert np.array_equal(a[k].astype(np.float64), want[k]), k
+
+
+def test_graph_replays_match_eager(gpu):
+    s = M.make_setup(120, 70, bc=0)
+    f = rand_fields(120, 70, 0, 11)
+    res = []
+    for spg in (0, 4):
+        b = Batch(PF_STORAGE_F64, pool_bytes=64 << 20)
+        j = b.add(make_desc(s, 100), f)
+        for r in range(5):                 # eager host work between replays
+            b.run(8, steps_per_graph=spg)
+            b.poll(j)
+        res.append(b.read(j))
+        b.close()
+    for k in range(4):
+        assert np.array_equal(res[0][k], res[1][k])
+
+
+def test_batch_nucleation_matches_plain(gpu):
+    """Nucleation schedule, random stream and embeds: identical to the plain
+    fused path, with P inside (0, 1)."""
+    for storage, dtype in ((PF_STORAGE_F64, torch.float64), (PF_STORAGE_F32, torch.float32)):
+        s = M.make_setup(64, 48, bc=1)
+        f = nuc_fields(64, 48, 1, 3)
+        b = Batch(storage, pool_bytes=64 << 20)
+        j = b.add(make_desc(s, 7, nucleation=True, seed=123, instance=4, nuc_every=3,
+                            nuc_dx=NUC_DX), f)
+        b.run(7)
+        got = b.read(j)
+        assert b.status() == 0
+        b.close()
+        want = plain(s, f, 7, dtype, nucleation=True, seed=123, instance=4, nuc_every=3,
+                     nuc_dx=NUC_DX)
+        nucleated = (np.asarray(want[2]) == 1.0) | (np.asarray(want[3]) == 1.0)
+        assert nucleated[1:-1, 1:-1].sum() > 50
+        for k in range(4):
+            assert np.array_equal(got[k].astype(np.float64), want[k]), (storage, k)
+
+
+def test_admission_by_pool_budget(gpu):
+    from pfpa.batch import job_bytes
+    per = job_bytes(PF_STORAGE_F32, 128, 128)
+    b = Batch(PF_STORAGE_F32, pool_bytes=per * 3 + 1000)
+    s = M.make_setup(128, 128, bc=1)
+    f = rand_fields(128, 128, 1, 1, particles=False)
+    ids = [b.add(make_desc(s, 5), f) for _ in range(3)]
+    assert None not in ids
+    assert b.add(make_desc(s, 5), f) is None      # no room
+    b.remove(ids[1])
+    j = b.add(make_desc(s, 5), f)                 # room again
+    assert j is not None
+    b.run(5)
+    assert all(not b.poll(i)[1] for i in (ids[0], ids[2], j))
+    b.close()
+
+
+def test_batch_summarize(gpu, oracle):
+    s = M.make_setup(200, 150, bc=0)
+    f = M.init_matrix_with_particles(200, 150, s.dx, 0, seed=2, radius=2e-9)
+    b = Batch(PF_STORAGE_F64, pool_bytes=64 << 20)
+    j = b.add(make_desc(s, 3), f)
+    b.run(3)
+    got = b.summarize(j)
+    want = oracle.summarize(b.read(j))
+    for g, w in zip(got, want):
+        assert g == pytest.approx(w, rel=1e-14, abs=1e-16)
+    b.close()
