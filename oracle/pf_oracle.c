@@ -476,9 +476,10 @@ static void orc_embed(double *xCr, double *xNb, double *phi, int nx, int ny,
  * dx_arg, dy_arg are what the kernel receives (the reference passes
  * meshres*nx and meshres*ny, :1206); dt_arg is nuc_interval*dt.
  * Deterministic two-phase form: all decisions are taken on the fields as
- * they are on entry, then embeds are applied in cell order (delta before
- * Laves).  For scan and embed radii below one cell (every BASELINE size)
- * this is identical to the reference kernel.
+ * they are on entry, then embeds are applied: those confined to their own
+ * cell first, then wider ones, each in cell order (delta before Laves).  For
+ * scan and embed radii below one cell (every realistic domain) this is
+ * identical to the reference kernel, which races on wider ones.
  * Returns the number of events; event_flags (optional, nx*ny bytes) gets
  * bit0 = delta event, bit1 = Laves event per cell.
  */
@@ -542,11 +543,18 @@ int orc_nucleation(int nx, int ny, double *xCr, double *xNb, double *pd, double 
 			}
 		}
 	}
-	for (int e = 0; e < nev; e++) {
-		const int x = ev[e].cell % nx, y = ev[e].cell / nx;
-		const int ph = ev[e].phase;
-		orc_embed(xCr, xNb, ph == 0 ? pd : pl, nx, ny, x, y, ev[e].xCr, ev[e].xNb,
-		          ORC_XE[1 + ph][0], ORC_XE[1 + ph][1], ev[e].R_precip);
+	/* apply: embeds confined to their own cell first, then wider ones; each
+	 * pass in (cell, phase) order, the order the decisions were recorded in */
+	for (int pass = 0; pass < 2; pass++) {
+		for (int e = 0; e < nev; e++) {
+			const int wide = ev[e].R_precip >= 1.0;
+			if (wide != pass)
+				continue;
+			const int x = ev[e].cell % nx, y = ev[e].cell / nx;
+			const int ph = ev[e].phase;
+			orc_embed(xCr, xNb, ph == 0 ? pd : pl, nx, ny, x, y, ev[e].xCr, ev[e].xNb,
+			          ORC_XE[1 + ph][0], ORC_XE[1 + ph][1], ev[e].R_precip);
+		}
 	}
 	free(ev);
 	return nev;

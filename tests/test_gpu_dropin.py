@@ -1,0 +1,110 @@
+"""The drop-in host API driven exactly as the reference's driver drives it
+(src/main.cpp:225-327), from C++ (tests/cpp/replay_main) and from Python
+(pfpa.api), checked against the oracle."""
+import os
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from pfpa import api
+from pfpa import model as M
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HARNESS = os.path.join(ROOT, "tests", "cpp", "replay_main")
+TOL = 1e-12
+
+
+def mixed_err(a, b):
+    a = a[1:-1, 1:-1]
+    b = b[1:-1, 1:-1]
+    sf = max(np.abs(b).max(), 1e-300)
+    return float((np.abs(a - b) / np.maximum(np.maximum(np.abs(a), np.abs(b)), sf)).max())
+
+
+def oracle_reference(oracle, f0, setup, steps):
+    P = oracle.make_params(setup.dt, setup.DCr, setup.DNb, setup.kappa, setup.omega,
+                           setup.Lmob, setup.alpha, setup.mask)
+    return oracle.run(f0, P, 0, steps)
+
+
+def test_cpp_driver_replay(gpu, oracle):
+    assert os.path.exists(HARNESS), "tests/cpp/replay_main not built: run __graft_entry__.build()"
+    Nx, Ny = 120, 90
+    s = M.make_setup(Nx, Ny, bc=0)
+    f0 = M.init_matrix_with_particles(Nx, Ny, s.dx, 0, seed=5, radius=2.5e-9)
+    steps = 300
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        fout = os.path.join(td, "out.bin")
+        with open(fin, "wb") as fh:
+            fh.write(struct.pack("ii", Nx + 2, Ny + 2))
+            for a in f0:
+                fh.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+        r = subprocess.run([HARNESS, fin, fout, str(steps), repr(s.dt), repr(s.dx), "100", "42",
+                            "150"], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        raw = np.fromfile(fout, dtype=np.uint8)
+    n = (Nx + 2) * (Ny + 2)
+    arr = raw[8:].view(np.float64).reshape(6, Ny + 2, Nx + 2)
+    want = oracle_reference(oracle, f0, s, steps)
+    for k in range(4):
+        assert mixed_err(arr[k], want[k]) <= TOL, k
+    # visualisation fields were produced at the last I/O step (step 300)
+    assert np.allclose(arr[4], 1.0 - arr[0] - arr[1], atol=1e-15)
+    p = lambda x: x ** 3 * (6 * x * x - 15 * x + 10)  # noqa: E731
+    assert np.allclose(arr[5], p(arr[2]) + p(arr[3]), rtol=1e-13, atol=1e-15)
+    assert n == arr[0].size
+
+
+def test_python_api_replay(gpu, oracle):
+    """pfpa.api in the reference's call sequence, I/O steps included."""
+    Nx, Ny = 100, 70
+    s = M.make_setup(Nx, Ny, bc=0)
+    f0 = M.init_matrix_with_particles(Nx, Ny, s.dx, 0, seed=8, radius=2e-9)
+    nx, ny, nm, bx, by = Nx + 2, Ny + 2, 3, 13, 13
+    host = api.HostData()
+    dev = api.CudaData()
+    api.make_arrays(host, nx, ny, nm)
+    api.five_point_Laplacian_stencil(s.dx, s.dx, host.mask_lap, nm)
+    host.conc_Cr_old[...] = f0[0]
+    host.conc_Nb_old[...] = f0[1]
+    host.phi_del_old[...] = f0[2]
+    host.phi_lav_old[...] = f0[3]
+    D_Cr, D_Nb, Lmob = M.set_diffusion_matrix(0.30, 0.02)
+    api.init_cuda(host, nx, ny, nm, M.kappa(), M.omega(), Lmob, D_Cr, D_Nb, dev)
+    api.device_init_prng(dev, nx, ny, nm, bx, by, seed=42)
+    steps, nuc_interval, io_interval = 200, 50, 64
+    for t in range(steps):
+        api.device_boundaries(dev, nx, ny, nm, bx, by)
+        api.device_fictitious(dev, nx, ny, nm, bx, by)
+        api.fictitious_boundaries(dev, nx, ny, nm, bx, by)
+        api.device_laplacian(dev, nx, ny, nm, bx, by, s.dx, s.dx)
+        api.device_laplacian_boundaries(dev, nx, ny, nm, bx, by)
+        api.device_evolution(dev, nx, ny, nm, bx, by, M.OMEGA_3PHASE, s.dt)
+        if t % nuc_interval == 0:
+            api.device_nucleation(dev, nx, ny, nm, bx, by, M.SIGMA[0], M.SIGMA[1],
+                                  M.LATTICE_CONST, M.IFCE_WIDTH, s.dx, s.dx, s.dx,
+                                  nuc_interval * s.dt)
+        dev.conc_Cr_old, dev.conc_Cr_new = api.swap_pointers_1D(dev.conc_Cr_old, dev.conc_Cr_new)
+        dev.conc_Nb_old, dev.conc_Nb_new = api.swap_pointers_1D(dev.conc_Nb_old, dev.conc_Nb_new)
+        dev.phi_del_old, dev.phi_del_new = api.swap_pointers_1D(dev.phi_del_old, dev.phi_del_new)
+        dev.phi_lav_old, dev.phi_lav_new = api.swap_pointers_1D(dev.phi_lav_old, dev.phi_lav_new)
+        if (t + 1) % io_interval == 0:
+            api.device_dataviz(dev, host, nx, ny, nm, bx, by)
+            api.read_out_result(dev, host, nx, ny)
+    api.read_out_result(dev, host, nx, ny)
+    summary = api.summarize_fields(dev, nx, ny)
+    assert api.nucleation_status(dev) == (0, False)
+    api.free_cuda(dev)
+    want = oracle_reference(oracle, f0, s, steps)
+    got = [host.conc_Cr_new, host.conc_Nb_new, host.phi_del_new, host.phi_lav_new]
+    for k in range(4):
+        assert mixed_err(got[k], want[k]) <= TOL, k
+    ws = oracle.summarize(want)
+    for a, b in zip(summary, ws):
+        assert a == pytest.approx(b, rel=1e-12, abs=1e-15)

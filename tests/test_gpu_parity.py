@@ -371,3 +371,57 @@ def test_errors_are_reported(gpu):
     with pytest.raises(pfpa.PfError):
         check(L.pf_thermo_eval(9, 1, _p(t), _p(t), None), "eval")
     L.pf_ctx_destroy(ctx)
+
+
+# ------------------------------------------------------------------ less common paths
+
+
+@pytest.mark.parametrize("bc", [0, 1])
+def test_nine_point_mask(gpu, oracle, bc):
+    """A nine-point Laplacian mask (src/mesh.cpp:198-213) selects the variant
+    that also shuffles the diagonal neighbours."""
+    Nx, Ny = 150, 77
+    dx = 0.25e-9
+    s = M.make_setup(Nx, Ny, bc=bc, dt_rule=5e-8)
+    m = [1 / (6 * dx * dx), 4 / (6 * dx * dx), 1 / (6 * dx * dx),
+         4 / (6 * dx * dx), -10 * (2 * dx * dx) / (6 * dx ** 4), 4 / (6 * dx * dx),
+         1 / (6 * dx * dx), 4 / (6 * dx * dx), 1 / (6 * dx * dx)]
+    s.mask = m
+    f0 = M.init_matrix_with_particles(Nx, Ny, dx, bc, seed=6, radius=2e-9)
+    sim = Simulation(s, f0)
+    sim.step(25)
+    got = sim.fields()
+    sim.close()
+    want = oracle.run(f0, oracle_params(oracle, s), bc, 25)
+    for k in range(4):
+        assert mixed_err(got[k], want[k]) <= TOL, k
+
+
+def test_wide_nucleation_events(gpu, oracle):
+    """Scan radius over the whole grid and embeds of radius ~1.7 cells that
+    overlap: the deferred-event path, in the oracle's order."""
+    Nx, Ny = 38, 30
+    nx, ny = Nx + 2, Ny + 2
+    rng = np.random.default_rng(21)
+    fields = [M.fill_ghosts(rng.uniform(0.15, 0.3, (ny, nx)), 0),
+              M.fill_ghosts(rng.uniform(0.19, 0.24, (ny, nx)), 0),
+              np.zeros((ny, nx)), np.zeros((ny, nx))]
+    a = dict(sigma_del=0.0105, sigma_lav=0.0108, unit_a=0.352e-9, dx=2e-13, dz=2e-13, dt=3e38)
+    want, nev, flags = oracle.nucleation(fields, a["sigma_del"], a["sigma_lav"], a["unit_a"],
+                                         a["dx"] * nx, a["dx"] * ny, a["dz"], a["dt"],
+                                         seed=77, call=2)
+    assert nev > 20
+    ctx = ctypes.c_void_p()
+    check(lib().pf_ctx_create(ctypes.byref(ctx)), "ctx")
+    d = [_dev(x) for x in fields]
+    N = _nuc_struct(a, nx, ny, 77, 2)
+    check(lib().pf_nucleation_f64(ctx, nx, ny, *[_p(x) for x in d], ctypes.byref(N), None),
+          "nucleation")
+    torch.cuda.synchronize()
+    for k in range(4):
+        assert np.array_equal(d[k].cpu().numpy(), want[k]), k
+    wide = ctypes.c_uint64()
+    over = ctypes.c_int()
+    check(lib().pf_nucleation_status(ctx, ctypes.byref(wide), ctypes.byref(over)), "status")
+    assert wide.value == nev and over.value == 0
+    lib().pf_ctx_destroy(ctx)
