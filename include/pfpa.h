@@ -93,6 +93,23 @@ int pf_step_f32(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm,
                 float* phi_del_new, float* phi_lav_new,
                 const pf_nuc_params* nuc, void* stream);
 
+/* Two fused steps, old -> new, without the intermediate level ever reaching
+ * memory (temporal blocking).  The result equals two pf_step_* calls without
+ * nucleation, the first writing a scratch level that the second reads: the
+ * reference's step sequence (src/main.cpp:289-305, 318-321) run twice.
+ * Five-point masks only; the interior must be at least 3x3.  Nucleation steps
+ * (src/main.cpp:307-316) go through pf_step_* one at a time. */
+int pf_step2_f64(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm,
+                 const double* conc_Cr_old, const double* conc_Nb_old,
+                 const double* phi_del_old, const double* phi_lav_old,
+                 double* conc_Cr_new, double* conc_Nb_new,
+                 double* phi_del_new, double* phi_lav_new, void* stream);
+int pf_step2_f32(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm,
+                 const float* conc_Cr_old, const float* conc_Nb_old,
+                 const float* phi_del_old, const float* phi_lav_old,
+                 float* conc_Cr_new, float* conc_Nb_new,
+                 float* phi_del_new, float* phi_lav_new, void* stream);
+
 /* Standalone nucleation pass in place on the given fields.  Replaces
  * device_nucleation (src/discretization.cu:1179-1209). */
 int pf_nucleation_f64(pf_ctx* ctx, int nx, int ny, double* conc_Cr, double* conc_Nb,
@@ -143,6 +160,11 @@ int pf_step_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream
 
 /* Heuristic rows-per-block the step kernel uses for an nx x ny grid. */
 int pf_default_rows(int nx, int ny);
+/* Tile of the two-step kernel: rows per chunk (0 = heuristic) and prefetch
+ * depth (2, 3, 6; 0 = 3).  pf_default_rows2 is its heuristic for storage
+ * PF_STORAGE_F64 or PF_STORAGE_F32. */
+int pf_ctx_set_tile2(pf_ctx* ctx, int rows, int depth);
+int pf_default_rows2(int nx, int ny, int storage);
 
 /* ------------------------------------------------------------------------
  * Batched form: many independent simulations of uneven size advanced by one

@@ -80,6 +80,30 @@ int step_t(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const T* i
 }
 
 template <typename T>
+int step2_t(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const T* iCr,
+            const T* iNb, const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd, T* opl,
+            void* stream) {
+  if (!ctx || !prm) return fail(PF_ERR_ARG, "null context or parameters");
+  if (int r = check_grid(nx, ny)) return r;
+  if (nx < 5 || ny < 5) return fail(PF_ERR_ARG, "two-step kernel needs at least 3x3 interior cells");
+  if (int r = check_ptrs(iCr, iNb, ipd, ipl)) return r;
+  if (int r = check_ptrs(oCr, oNb, opd, opl)) return r;
+  if (bc != PF_BC_NOFLUX && bc != PF_BC_PERIODIC) return fail(PF_ERR_ARG, "unknown boundary mode");
+  if ((const void*)iCr == (const void*)oCr || (const void*)iNb == (const void*)oNb ||
+      (const void*)ipd == (const void*)opd || (const void*)ipl == (const void*)opl)
+    return fail(PF_ERR_ARG, "step cannot run in place: old and new buffers must differ");
+  const double* mk = prm->mask;
+  if (mk[0] != 0.0 || mk[2] != 0.0 || mk[6] != 0.0 || mk[8] != 0.0)
+    return fail(PF_ERR_ARG, "two-step kernel takes five-point masks only");
+  const pf::StepParams P = pf::make_step(prm->dt, prm->DCr, prm->DNb, prm->kappa, prm->omega,
+                                         prm->Lmob, prm->alpha, prm->mask);
+  hipError_t e = pf::launch_step2<T>(nx, ny, bc, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl,
+                                     ctx->tile, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "two-step kernel");
+  return PF_OK;
+}
+
+template <typename T>
 int nucleation_t(pf_ctx* ctx, int nx, int ny, T* xCr, T* xNb, T* pd, T* pl,
                  const pf_nuc_params* nuc, void* stream) {
   if (!ctx || !nuc) return fail(PF_ERR_ARG, "null context or parameters");
@@ -153,6 +177,32 @@ int pf_ctx_set_tile(pf_ctx* ctx, int rows, int xcd_remap, int depth) {
 }
 
 int pf_default_rows(int nx, int ny) { return pf::default_rows(nx, ny); }
+
+int pf_ctx_set_tile2(pf_ctx* ctx, int rows, int depth) {
+  if (!ctx) return fail(PF_ERR_ARG, "null context");
+  if (rows < 0 || rows > 1 << 20) return fail(PF_ERR_ARG, "rows out of range");
+  if (depth != 0 && depth != 2 && depth != 3 && depth != 6)
+    return fail(PF_ERR_ARG, "prefetch depth must be 2, 3 or 6 (0 = default)");
+  ctx->tile.rows2 = rows;
+  ctx->tile.depth2 = depth ? depth : 3;
+  return PF_OK;
+}
+
+int pf_default_rows2(int nx, int ny, int storage) {
+  return pf::default_rows2(nx, ny, storage == PF_STORAGE_F32 ? 4 : 8);
+}
+
+int pf_step2_f64(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const double* a,
+                 const double* b, const double* c, const double* d, double* oa, double* ob,
+                 double* oc, double* od, void* stream) {
+  return step2_t<double>(ctx, nx, ny, bc, prm, a, b, c, d, oa, ob, oc, od, stream);
+}
+
+int pf_step2_f32(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const float* a,
+                 const float* b, const float* c, const float* d, float* oa, float* ob,
+                 float* oc, float* od, void* stream) {
+  return step2_t<float>(ctx, nx, ny, bc, prm, a, b, c, d, oa, ob, oc, od, stream);
+}
 
 int pf_step_f64(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const double* a,
                 const double* b, const double* c, const double* d, double* oa, double* ob,

@@ -181,6 +181,26 @@ __device__ __forceinline__ void laplacians(const double* m, const Point& S, cons
   }
 }
 
+// The explicit Euler update of one cell from its stencil Laplacians:
+// Cahn-Hilliard (src/discretization.cu:859-863) and Allen-Cahn
+// (src/discretization.cu:888-942) with the closed-form grand-potential
+// difference at the solved potential.
+__device__ __forceinline__ void update_cell(const StepParams& P, const double lap[4],
+                                            const Point& cur, const Centre& cc, double& nCr,
+                                            double& nNb, double& npd, double& npl) {
+  nCr = cc.xCr + P.dt * (P.DCr0 * lap[2] + P.DCr1 * lap[3]);
+  nNb = cc.xNb + P.dt * (P.DNb0 * lap[2] + P.DNb1 * lap[3]);
+  const double fd = cur.pd, fl = cur.pl;
+  const double dOd = dOmega_del(cc.mu0, cc.mu1);
+  const double dOl = dOmega_lav(cc.mu0, cc.mu1);
+  const double dFd = -pPrime(fd) * dOd + 2.0 * P.w0 * fd * (fd - 1.0) * (2.0 * fd - 1.0) +
+                     2.0 * P.alpha * fd * fl * fl - P.k0 * lap[0];
+  const double dFl = -pPrime(fl) * dOl + 2.0 * P.w1 * fl * (fl - 1.0) * (2.0 * fl - 1.0) +
+                     2.0 * P.alpha * fl * fd * fd - P.k1 * lap[1];
+  npd = fd - P.dt * P.L0 * dFd;
+  npl = fl - P.dt * P.L1 * dFl;
+}
+
 template <typename T>
 __device__ __forceinline__ void store4(T* oCr, T* oNb, T* opd, T* opl, size_t o, double a,
                                        double b, double c, double d) {
@@ -299,22 +319,8 @@ __device__ __forceinline__ void march(
       laplacians<NINE>(m, win[ps], W, win[cs], E, win[ns], winW[ps], winE[ps], winW[ns],
                        winE[ns], lap);
 
-      const Point& cur = win[cs];
-      const Centre& cc = cen[cs];
-      // Cahn-Hilliard (src/discretization.cu:859-863)
-      double nCr = cc.xCr + P.dt * (P.DCr0 * lap[2] + P.DCr1 * lap[3]);
-      double nNb = cc.xNb + P.dt * (P.DNb0 * lap[2] + P.DNb1 * lap[3]);
-      // Allen-Cahn (src/discretization.cu:888-942) with the closed-form
-      // grand-potential difference at the solved potential
-      const double fd = cur.pd, fl = cur.pl;
-      const double dOd = dOmega_del(cc.mu0, cc.mu1);
-      const double dOl = dOmega_lav(cc.mu0, cc.mu1);
-      const double dFd = -pPrime(fd) * dOd + 2.0 * P.w0 * fd * (fd - 1.0) * (2.0 * fd - 1.0) +
-                         2.0 * P.alpha * fd * fl * fl - P.k0 * lap[0];
-      const double dFl = -pPrime(fl) * dOl + 2.0 * P.w1 * fl * (fl - 1.0) * (2.0 * fl - 1.0) +
-                         2.0 * P.alpha * fl * fd * fd - P.k1 * lap[1];
-      double npd = fd - P.dt * P.L0 * dFd;
-      double npl = fl - P.dt * P.L1 * dFl;
+      double nCr, nNb, npd, npl;
+      update_cell(P, lap, win[cs], cen[cs], nCr, nNb, npd, npl);
 
       const int py = 1 + y;
       if (NUC && out_lane) {
@@ -329,6 +335,145 @@ __device__ __forceinline__ void march(
       if (out_lane) {
         const int qx = 1 + gx;
         store4(oCr, oNb, opd, opl, (size_t)py * pitch + qx, nCr, nNb, npd, npl);
+        store_ghost_images<T, BC>(oCr, oNb, opd, opl, gx, y, Nx, Ny, pitch, nCr, nNb, npd, npl);
+      }
+    }
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// Two explicit steps per pass (temporal blocking).  The reference advances
+// one step per sequence of launches and so reads and writes every field once
+// per step; here a wave marches its strip through two time levels at once, so
+// the old fields are read once and the fields two steps later written once:
+// half the HBM traffic per step of the one-step march.
+//
+// A strip has 60 output columns and a two-column halo each side (lanes 0-1
+// and 62-63).  Stage A (the first step) is computed on every lane and is valid
+// on lanes 1..62; stage B (the second step) reads stage A's values through the
+// same DPP shifts and is valid, and stored, on lanes 2..61.  In y, stage A runs
+// one row ahead of stage B, from row y0-1 to row y1, so the chunk reads old
+// rows y0-2 .. y1+1.
+//
+// The intermediate level is exactly what a first one-step launch would have
+// stored: stage-A values are rounded to the storage type before stage B uses
+// them, and stage-A values at positions outside the interior are the images
+// the one-step kernel writes into the ghost ring.  Periodic images come out
+// of the index map directly (a halo lane at gx = -1 reads the neighbourhood of
+// column Nx-1).  No-flux images are copies of the adjacent interior cell's
+// stage-A value: taken from the neighbouring lane in x, and from the adjacent
+// stage-A row in y.  So two steps here give the same result as two one-step
+// launches.  Five-point masks, no nucleation (the caller runs nucleation
+// steps one at a time).
+constexpr int STRIP2 = WAVE - 4;  // output columns per two-step strip
+
+template <typename T, int BC, int DEPTH>
+__device__ __forceinline__ void march2(
+    const T* __restrict__ iCr, const T* __restrict__ iNb, const T* __restrict__ ipd,
+    const T* __restrict__ ipl, T* __restrict__ oCr, T* __restrict__ oNb, T* __restrict__ opd,
+    T* __restrict__ opl, int Nx, int Ny, int pitch, int strip, int chunk, int rows,
+    const StepParams& P) {
+  static_assert(UNROLL % DEPTH == 0, "ring depth must divide the unroll factor");
+  const int lane = threadIdx.x & (WAVE - 1);
+
+  const int gx = strip * STRIP2 + lane - 2;  // interior column of this lane (may be a halo)
+  const int px = 1 + bc_map<BC>(gx, Nx);
+  const bool out_lane = (lane >= 2) && (lane <= STRIP2 + 1) && (gx < Nx);
+  // does this wave hold a no-flux image column (gx = -1 or gx = Nx)?
+  const bool x_images = BC == 0 && (strip == 0 || strip * STRIP2 + WAVE - 3 >= Nx);
+
+  const int y0 = chunk * rows;
+  const int y1 = min(y0 + rows, Ny);
+  if (y0 >= Ny) return;
+
+  auto load = [&](Raw<T>& r, int y) {
+    const size_t a = (size_t)(1 + bc_map<BC>(y, Ny)) * (size_t)pitch + (size_t)px;
+    r.cr = iCr[a];
+    r.nb = iNb[a];
+    r.pd = ipd[a];
+    r.pl = ipl[a];
+  };
+  const double* m = P.m;
+
+  // Stage A at the centre of three derived old rows: the first step's values,
+  // rounded to storage, with no-flux x images, derived for stage B.
+  auto stage_a = [&](const Point& Sth, const Point& C, const Centre& cc, const Point& Nth,
+                     Point& v, Centre& c) {
+    const Point W = shift_left(C), E = shift_right(C);
+    double lap[4];
+    laplacians<false>(m, Sth, W, C, E, Nth, Sth, Sth, Nth, Nth, lap);
+    double n0, n1, n2, n3;
+    update_cell(P, lap, C, cc, n0, n1, n2, n3);
+    Raw<T> r;
+    r.cr = (T)n0;
+    r.nb = (T)n1;
+    r.pd = (T)n2;
+    r.pl = (T)n3;
+    if (x_images) {
+      const bool lo = gx == -1, hi = gx == Nx;
+      const double c0 = (double)r.cr, c1 = (double)r.nb, c2 = (double)r.pd, c3 = (double)r.pl;
+      const double fr0 = from_right(c0), fr1 = from_right(c1), fr2 = from_right(c2),
+                   fr3 = from_right(c3);
+      const double fl0 = from_left(c0), fl1 = from_left(c1), fl2 = from_left(c2),
+                   fl3 = from_left(c3);
+      r.cr = (T)(lo ? fr0 : (hi ? fl0 : c0));
+      r.nb = (T)(lo ? fr1 : (hi ? fl1 : c1));
+      r.pd = (T)(lo ? fr2 : (hi ? fl2 : c2));
+      r.pl = (T)(lo ? fr3 : (hi ? fl3 : c3));
+    }
+    derive<T>(r, v, c);
+  };
+
+  // old rows: row r in slot (r - y0 + 2) % 3; stage-A rows: row r in slot (r - y0 + 1) % 3
+  Point a[3], b[3];
+  Centre ac[3], bcn[3];
+  Raw<T> ring[DEPTH];  // old rows y0+2+j
+  {
+    Raw<T> r;
+    load(r, y0 - 2);
+    derive<T>(r, a[0], ac[0]);
+    load(r, y0 - 1);
+    derive<T>(r, a[1], ac[1]);
+    load(r, y0);
+    derive<T>(r, a[2], ac[2]);
+#pragma unroll
+    for (int j = 0; j < DEPTH; j++) load(ring[j], min(y0 + 2 + j, y1 + 1));
+    stage_a(a[0], a[1], ac[1], a[2], b[0], bcn[0]);  // stage-A row y0-1
+    load(r, y0 + 1);
+    derive<T>(r, a[0], ac[0]);
+    stage_a(a[1], a[2], ac[2], a[0], b[1], bcn[1]);  // stage-A row y0
+    if (BC == 0 && y0 == 0) {  // no-flux image row -1 = row 0
+      b[0] = b[1];
+      bcn[0] = bcn[1];
+    }
+  }
+
+  for (int yb = y0;; yb += UNROLL) {
+#pragma unroll
+    for (int k = 0; k < UNROLL; k++) {
+      const int y = yb + k;  // output row of stage B
+      if (y >= y1) return;
+      const int a0 = (k + 2) % 3, a1 = k % 3, a2 = (k + 1) % 3;  // old rows y, y+1, y+2
+      const int b0 = k % 3, b1 = (k + 1) % 3, b2 = (k + 2) % 3;  // stage-A rows y-1, y, y+1
+      const int rs = k % DEPTH;
+      derive<T>(ring[rs], a[a2], ac[a2]);
+      load(ring[rs], min(y + 2 + DEPTH, y1 + 1));
+
+      stage_a(a[a0], a[a1], ac[a1], a[a2], b[b2], bcn[b2]);  // stage-A row y+1
+      if (BC == 0 && y + 1 == Ny) {  // no-flux image row Ny = row Ny-1
+        b[b2] = b[b1];
+        bcn[b2] = bcn[b1];
+      }
+
+      const Point W = shift_left(b[b1]), E = shift_right(b[b1]);
+      double lap[4];
+      laplacians<false>(m, b[b0], W, b[b1], E, b[b2], b[b0], b[b0], b[b2], b[b2], lap);
+      double nCr, nNb, npd, npl;
+      update_cell(P, lap, b[b1], bcn[b1], nCr, nNb, npd, npl);
+
+      if (out_lane) {
+        store4(oCr, oNb, opd, opl, (size_t)(1 + y) * pitch + 1 + gx, nCr, nNb, npd, npl);
         store_ghost_images<T, BC>(oCr, oNb, opd, opl, gx, y, Nx, Ny, pitch, nCr, nNb, npd, npl);
       }
     }

@@ -101,6 +101,76 @@ hipError_t launch_step(int nx, int ny, int bc, const StepParams& P, const T* iCr
 #undef PF_LAUNCH
 }
 
+// ---- two steps per launch (march2) ----
+
+template <typename T, int BC, int DEPTH>
+__global__ __launch_bounds__(WAVE * WPB) void step2_kernel(
+    const T* __restrict__ iCr, const T* __restrict__ iNb, const T* __restrict__ ipd,
+    const T* __restrict__ ipl, T* __restrict__ oCr, T* __restrict__ oNb, T* __restrict__ opd,
+    T* __restrict__ opl, int Nx, int Ny, int pitch, int strips, int chunks, int rows, int remap,
+    StepParams P) {
+  const int nwaves = strips * chunks;
+  const int nblocks = (nwaves + WPB - 1) / WPB;
+  const int b = (remap ? xcd_remap(blockIdx.x, nblocks) : (int)blockIdx.x) * WPB +
+                (int)(threadIdx.x / WAVE);
+  if (b >= nwaves) return;
+  march2<T, BC, DEPTH>(iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny, pitch, b % strips,
+                       b / strips, rows, P);
+}
+
+int default_rows2(int nx, int ny, int storage_bytes) {
+  // Stage A recomputes two rows per chunk, so chunks are longer than the
+  // one-step kernel's while the grid still fills the chip.  FP64 storage is
+  // near the HBM bound and prefers more waves in flight (16 rows at 4096^2);
+  // FP32 storage is bound by arithmetic and prefers less recomputation (32).
+  const int Nx = nx - 2, Ny = ny - 2;
+  const int strips = (Nx + STRIP2 - 1) / STRIP2;
+  int rows = storage_bytes == 8 ? 16 : 32;
+  while (rows > 6 && (long)strips * ((Ny + rows - 1) / rows) < 8192) rows >>= 1;
+  return rows;
+}
+
+template <typename T, int BC, int DEPTH>
+static hipError_t launch_step2_t(int nx, int ny, const StepParams& P, const T* iCr,
+                                 const T* iNb, const T* ipd, const T* ipl, T* oCr, T* oNb,
+                                 T* opd, T* opl, const TileConfig& tc, hipStream_t st) {
+  const int Nx = nx - 2, Ny = ny - 2;
+  const int rows = tc.rows2 > 0 ? tc.rows2 : default_rows2(nx, ny, (int)sizeof(T));
+  const int strips = (Nx + STRIP2 - 1) / STRIP2;
+  const int chunks = (Ny + rows - 1) / rows;
+  const int blocks = (strips * chunks + WPB - 1) / WPB;
+  hipLaunchKernelGGL((step2_kernel<T, BC, DEPTH>), dim3(blocks), dim3(WAVE * WPB), 0, st, iCr,
+                     iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny, nx, strips, chunks, rows,
+                     tc.remap, P);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_step2(int nx, int ny, int bc, const StepParams& P, const T* iCr, const T* iNb,
+                        const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd, T* opl,
+                        const TileConfig& tc, hipStream_t st) {
+#define PF_LAUNCH2(BCV, D)                                                                  \
+  return launch_step2_t<T, BCV, D>(nx, ny, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, tc, st)
+  if (bc == 0) {
+    if (tc.depth2 == 2) PF_LAUNCH2(0, 2);
+    if (tc.depth2 == 6) PF_LAUNCH2(0, 6);
+    PF_LAUNCH2(0, 3);
+  } else {
+    if (tc.depth2 == 2) PF_LAUNCH2(1, 2);
+    if (tc.depth2 == 6) PF_LAUNCH2(1, 6);
+    PF_LAUNCH2(1, 3);
+  }
+#undef PF_LAUNCH2
+}
+
+template hipError_t launch_step2<double>(int, int, int, const StepParams&, const double*,
+                                         const double*, const double*, const double*, double*,
+                                         double*, double*, double*, const TileConfig&,
+                                         hipStream_t);
+template hipError_t launch_step2<float>(int, int, int, const StepParams&, const float*,
+                                        const float*, const float*, const float*, float*, float*,
+                                        float*, float*, const TileConfig&, hipStream_t);
+
 template hipError_t launch_step<double>(int, int, int, const StepParams&, const double*,
                                         const double*, const double*, const double*, double*,
                                         double*, double*, double*, const NucParams*,

@@ -56,6 +56,10 @@ PROTOTYPES = {
                     [ctypes.POINTER(PfNucParams), _vp]),
     "pf_step_f32": (_i, [_vp, _i, _i, _i, ctypes.POINTER(PfParams)] + [_vp] * 8 +
                     [ctypes.POINTER(PfNucParams), _vp]),
+    "pf_step2_f64": (_i, [_vp, _i, _i, _i, ctypes.POINTER(PfParams)] + [_vp] * 8 + [_vp]),
+    "pf_step2_f32": (_i, [_vp, _i, _i, _i, ctypes.POINTER(PfParams)] + [_vp] * 8 + [_vp]),
+    "pf_ctx_set_tile2": (_i, [_vp, _i, _i]),
+    "pf_default_rows2": (_i, [_i, _i, _i]),
     "pf_nucleation_f64": (_i, [_vp, _i, _i] + [_vp] * 4 + [ctypes.POINTER(PfNucParams), _vp]),
     "pf_nucleation_f32": (_i, [_vp, _i, _i] + [_vp] * 4 + [ctypes.POINTER(PfNucParams), _vp]),
     "pf_nucleation_status": (_i, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_i)]),

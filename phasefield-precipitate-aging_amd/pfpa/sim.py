@@ -1,6 +1,8 @@
-"""Native stepping loop over the C-ABI: one fused launch per step, nucleation
-fused into the steps the reference's schedule selects
-(kernel_time % nuc_interval == 0, src/main.cpp:307-316)."""
+"""Native stepping loop over the C-ABI.  Steps run two per launch (the
+temporally blocked kernel) wherever neither step of the pair is a nucleation
+step; nucleation steps, the ones the reference's schedule selects
+(kernel_time % nuc_interval == 0, src/main.cpp:307-316), run one per launch
+with the nucleation pass fused."""
 import ctypes
 
 import numpy as np
@@ -21,7 +23,8 @@ class Simulation:
     (FP32 storage, FP64 arithmetic)."""
 
     def __init__(self, setup, fields, dtype=torch.float64, nucleation=False, seed=42,
-                 nuc_every=None, instance=0, stream=None, ctx=None, nuc_dx=None):
+                 nuc_every=None, instance=0, stream=None, ctx=None, nuc_dx=None,
+                 fuse_steps=True):
         if not torch.cuda.is_available():
             raise RuntimeError("Simulation needs a GPU")
         self.s = setup
@@ -65,25 +68,46 @@ class Simulation:
         self.N = N
         self.steps_done = 0
         self._step = lib().pf_step_f64 if dtype == torch.float64 else lib().pf_step_f32
+        self._step2 = lib().pf_step2_f64 if dtype == torch.float64 else lib().pf_step2_f32
+        five_point = all(setup.mask[k] == 0.0 for k in (0, 2, 6, 8))
+        self.fuse_steps = fuse_steps and five_point and self.nx >= 5 and self.ny >= 5
         self._summ = lib().pf_summarize_f64 if dtype == torch.float64 else lib().pf_summarize_f32
 
     def set_tile(self, rows, remap=1, depth=0):
         check(lib().pf_ctx_set_tile(self.ctx, rows, remap, depth), "pf_ctx_set_tile")
 
+    def set_tile2(self, rows, depth=0):
+        check(lib().pf_ctx_set_tile2(self.ctx, rows, depth), "pf_ctx_set_tile2")
+
+    def _nuc_due(self, t):
+        return self.nucleation and t % self.nuc_every == 0
+
     def step(self, n=1):
         st = ctypes.c_void_p(self.stream.cuda_stream)
-        for _ in range(n):
-            nuc = None
-            if self.nucleation and self.steps_done % self.nuc_every == 0:
-                self.N.call = self.steps_done // self.nuc_every
-                nuc = ctypes.byref(self.N)
+        left = n
+        while left > 0:
             a, b = self.a, self.b
-            check(self._step(self.ctx, self.nx, self.ny, self.s.bc, ctypes.byref(self.P),
-                             _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(a[3]),
-                             _ptr(b[0]), _ptr(b[1]), _ptr(b[2]), _ptr(b[3]), nuc, st),
-                  "pf_step")
+            t = self.steps_done
+            if (self.fuse_steps and left >= 2 and not self._nuc_due(t)
+                    and not self._nuc_due(t + 1)):
+                check(self._step2(self.ctx, self.nx, self.ny, self.s.bc, ctypes.byref(self.P),
+                                  _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(a[3]),
+                                  _ptr(b[0]), _ptr(b[1]), _ptr(b[2]), _ptr(b[3]), st),
+                      "pf_step2")
+                done = 2
+            else:
+                nuc = None
+                if self._nuc_due(t):
+                    self.N.call = t // self.nuc_every
+                    nuc = ctypes.byref(self.N)
+                check(self._step(self.ctx, self.nx, self.ny, self.s.bc, ctypes.byref(self.P),
+                                 _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(a[3]),
+                                 _ptr(b[0]), _ptr(b[1]), _ptr(b[2]), _ptr(b[3]), nuc, st),
+                      "pf_step")
+                done = 1
             self.a, self.b = self.b, self.a
-            self.steps_done += 1
+            self.steps_done += done
+            left -= done
 
     def fields(self):
         self.stream.synchronize()

@@ -1,0 +1,145 @@
+"""The two-step (temporally blocked) kernel, pf_step2_*, through the C-ABI.
+
+It must give what two one-step launches give: the same interior values and
+the same ghost-ring images, for both boundary modes, both storage types,
+ragged grids and every tile shape.  Its parity with the oracle over long runs
+is covered by the Simulation-driven tests, which pair steps through it.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from pfpa import model as M
+from pfpa._lib import PfParams, check, lib
+from pfpa.sim import Simulation
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _params(s, mask=None):
+    P = PfParams()
+    P.dt = s.dt
+    for k in range(2):
+        P.DCr[k], P.DNb[k] = s.DCr[k], s.DNb[k]
+        P.kappa[k], P.omega[k], P.Lmob[k] = s.kappa[k], s.omega[k], s.Lmob[k]
+    P.alpha = s.alpha
+    m = s.mask if mask is None else mask
+    for k in range(9):
+        P.mask[k] = m[k]
+    return P
+
+
+def _fields(Nx, Ny, bc, seed):
+    rng = np.random.default_rng(seed)
+    xCr = 0.3 + 0.01 * rng.standard_normal((Ny + 2, Nx + 2))
+    xNb = 0.02 + 0.005 * rng.standard_normal((Ny + 2, Nx + 2))
+    pd = np.clip(rng.uniform(-0.3, 1.3, (Ny + 2, Nx + 2)), 0, 1)
+    pl = np.clip(rng.uniform(-0.5, 1.0, (Ny + 2, Nx + 2)), 0, 1) * (pd < 0.3)
+    return [M.fill_ghosts(a, bc) for a in (xCr, xNb, pd, pl)]
+
+
+def _mixed_err(a, b):
+    sf = max(np.abs(b).max(), 1e-300)
+    return float((np.abs(a - b) / np.maximum(np.maximum(np.abs(a), np.abs(b)), sf)).max())
+
+
+@pytest.fixture(scope="module")
+def ctx(gpu):
+    h = ctypes.c_void_p()
+    check(lib().pf_ctx_create(ctypes.byref(h)), "pf_ctx_create")
+    yield h
+    torch.cuda.synchronize()
+    lib().pf_ctx_destroy(h)
+
+
+SHAPES = [(3, 3), (4, 5), (5, 4), (59, 7), (60, 61), (61, 62), (62, 3), (120, 9), (121, 33),
+          (200, 131), (37, 300)]
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("bc", [0, 1])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_two_steps_in_one_launch(ctx, shape, bc, dtype):
+    Nx, Ny = shape
+    nx, ny = Nx + 2, Ny + 2
+    s = M.make_setup(Nx, Ny, bc=bc, dt_rule=4e-7)
+    P = _params(s)
+    f0 = [torch.as_tensor(f).to(device="cuda", dtype=dtype) for f in
+          _fields(Nx, Ny, bc, 1000 * Nx + Ny)]
+    mid = [torch.full_like(x, np.nan) for x in f0]
+    ref = [torch.full_like(x, np.nan) for x in f0]
+    one = lib().pf_step_f64 if dtype == torch.float64 else lib().pf_step_f32
+    two = lib().pf_step2_f64 if dtype == torch.float64 else lib().pf_step2_f32
+    check(one(ctx, nx, ny, bc, ctypes.byref(P), *map(_p, f0), *map(_p, mid), None, None), "step")
+    check(one(ctx, nx, ny, bc, ctypes.byref(P), *map(_p, mid), *map(_p, ref), None, None), "step")
+    want = [x.double().cpu().numpy() for x in ref]
+    outs = []
+    for rows, depth in ((0, 0), (1, 2), (5, 6), (6, 3), (7, 2), (64, 3)):
+        check(lib().pf_ctx_set_tile2(ctx, rows, depth), "tile2")
+        out = [torch.full_like(x, np.nan) for x in f0]
+        check(two(ctx, nx, ny, bc, ctypes.byref(P), *map(_p, f0), *map(_p, out), None), "step2")
+        got = [x.double().cpu().numpy() for x in out]
+        for k in range(4):
+            # interior and ghost ring alike
+            assert np.all(np.isfinite(got[k])), (rows, depth, k)
+            assert _mixed_err(got[k], want[k]) <= (TOL if dtype == torch.float64 else 0.0), \
+                (rows, depth, k, _mixed_err(got[k], want[k]))
+        outs.append(got)
+    check(lib().pf_ctx_set_tile2(ctx, 0, 0), "tile2")
+    for o in outs[1:]:
+        for k in range(4):
+            assert np.array_equal(o[k], outs[0][k])
+
+
+@pytest.mark.parametrize("bc", [0, 1])
+def test_fused_run_matches_one_step_run(gpu, oracle, bc):
+    """A Simulation pairing steps through the two-step kernel, with the
+    reference's nucleation schedule (odd interval, so pairs straddle the
+    nucleation steps), against one launch per step and against the oracle."""
+    Nx, Ny = 150, 97
+    s = M.make_setup(Nx, Ny, bc=bc)
+    f0 = M.init_matrix_with_particles(Nx, Ny, s.dx, bc, seed=3, radius=2.5e-9)
+    a = Simulation(s, f0, nucleation=True, nuc_every=7)
+    b = Simulation(s, f0, nucleation=True, nuc_every=7, fuse_steps=False)
+    assert a.fuse_steps and not b.fuse_steps
+    for n in (1, 2, 5, 40, 153):
+        a.step(n)
+        b.step(n)
+    ga, gb = a.fields(), b.fields()
+    want = oracle.run(f0, oracle.make_params(s.dt, s.DCr, s.DNb, s.kappa, s.omega, s.Lmob,
+                                             s.alpha, s.mask), bc, a.steps_done)
+    for k in range(4):
+        assert _mixed_err(ga[k], gb[k]) <= TOL, k
+        assert _mixed_err(ga[k][1:-1, 1:-1], want[k][1:-1, 1:-1]) <= TOL, k
+    a.close()
+    b.close()
+
+
+def test_step2_errors(ctx):
+    s = M.make_setup(8, 8, bc=0)
+    P = _params(s)
+    f = [torch.zeros(10 * 10, dtype=torch.float64, device="cuda") for _ in range(8)]
+    two = lib().pf_step2_f64
+    # in place
+    assert two(ctx, 10, 10, 0, ctypes.byref(P), *map(_p, f[:4]), *map(_p, f[:4]), None) != 0
+    # nine-point mask
+    m9 = list(s.mask)
+    m9[0] = m9[2] = m9[6] = m9[8] = 1.0
+    P9 = _params(s, m9)
+    assert two(ctx, 10, 10, 0, ctypes.byref(P9), *map(_p, f[:4]), *map(_p, f[4:]), None) != 0
+    assert b"five-point" in lib().pf_last_error()
+    # interior smaller than 3x3
+    assert two(ctx, 4, 10, 0, ctypes.byref(P), *map(_p, f[:4]), *map(_p, f[4:]), None) != 0
+    # bad boundary mode, bad tile
+    assert two(ctx, 10, 10, 7, ctypes.byref(P), *map(_p, f[:4]), *map(_p, f[4:]), None) != 0
+    assert lib().pf_ctx_set_tile2(ctx, 8, 4) != 0
+    assert two(ctx, 10, 10, 0, ctypes.byref(P), *map(_p, f[:4]), *map(_p, f[4:]), None) == 0
+    torch.cuda.synchronize()
