@@ -61,18 +61,17 @@ StepParams make_step(double dt, const double* DCr, const double* DNb, const doub
                      const double* omega, const double* Lmob, double alpha,
                      const double* mask) {
   StepParams P{};
-  P.dt = dt;
-  P.DCr0 = DCr[0];
-  P.DCr1 = DCr[1];
-  P.DNb0 = DNb[0];
-  P.DNb1 = DNb[1];
-  P.L0 = Lmob[0];
-  P.L1 = Lmob[1];
-  P.k0 = kappa[0];
-  P.k1 = kappa[1];
-  P.w0 = omega[0];
-  P.w1 = omega[1];
-  P.alpha = alpha;
+  P.cCr0 = dt * DCr[0];
+  P.cCr1 = dt * DCr[1];
+  P.cNb0 = dt * DNb[0];
+  P.cNb1 = dt * DNb[1];
+  for (int i = 0; i < 2; i++) {
+    const double dtL = dt * Lmob[i];
+    P.a[i] = 30.0 * dtL;
+    P.w[i] = 2.0 * omega[i] * dtL;
+    P.c[i] = 2.0 * alpha * dtL;
+    P.k[i] = kappa[i] * dtL;
+  }
   for (int i = 0; i < 9; i++) P.m[i] = mask[i];
   return P;
 }

@@ -60,7 +60,7 @@ __device__ __forceinline__ void xcd_range(int total, int& first, int& last, int&
   first = (int)(((long)total * x) >> 3);
   last = (int)(((long)total * (x + 1)) >> 3);
   stride = per_xcd_blocks * WPB;
-  start = first + lb * WPB + (int)(threadIdx.x / WAVE);
+  start = first + lb * WPB + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
 }
 
 template <typename T, int DEPTH>

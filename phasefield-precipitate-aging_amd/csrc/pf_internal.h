@@ -9,14 +9,19 @@
 
 namespace pf {
 
-// Kinetic and discretisation parameters of one step (by value into kernels).
+// Kinetic and discretisation parameters of one step (by value into kernels),
+// with dt and the mobilities folded in on the host:
+//   x_Cr' = x_Cr + cCr0 lap(gCr) + cCr1 lap(gNb)            (dt * D_Cr row)
+//   phi'  = phi + dtL [30 u^2 dOmega + 2 omega u (2 phi - 1)
+//                      - 2 alpha phi phi_o^2 + kappa lap(phi)],  u = phi (1 - phi)
+// which is src/discretization.cu:859-863 and 888-942 with
+// p'(phi) = 30 u^2 and phi (phi - 1)(2 phi - 1) = -u (2 phi - 1).
 struct StepParams {
-  double dt;
-  double DCr0, DCr1, DNb0, DNb1;  // constant diffusivity matrix rows
-  double L0, L1;                  // Allen-Cahn mobilities (delta, Laves)
-  double k0, k1;                  // gradient energy coefficients kappa
-  double w0, w1;                  // double-well heights omega
-  double alpha;                   // three-phase coefficient Omega
+  double cCr0, cCr1, cNb0, cNb1;  // dt * D rows
+  double a[2];                    // 30 dt L        (delta, Laves)
+  double w[2];                    // 2 omega dt L
+  double c[2];                    // 2 alpha dt L
+  double k[2];                    // kappa dt L
   double m[9];                    // 3x3 Laplacian mask, row-major (y-1, y, y+1)
 };
 

@@ -50,15 +50,16 @@ __device__ __forceinline__ int bc_map(int i, int n) {
 }
 
 // DPP wave shifts (GFX9 wave_shr:1 / wave_shl:1): lane i receives lane i-1 /
-// lane i+1.  Lanes 0 and 63 receive 0; their results are never stored.
+// lane i+1.  With bound_ctrl set, lanes 0 and 63 (no source lane) receive 0;
+// their results are never stored.
 __device__ __forceinline__ double from_left(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double from_right(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
 
@@ -75,18 +76,77 @@ struct Raw {
   T cr, nb, pd, pl;
 };
 
+// Field access split into a wave-uniform row byte offset (kept in scalar
+// registers) and a 32-bit per-lane byte offset, so that loads and stores take
+// the scalar-base + vector-offset form with no per-lane 64-bit address math.
+template <typename T>
+__device__ __forceinline__ T ld_at(const T* base, size_t row_bytes, uint32_t lane_bytes) {
+  return *(const T*)((const char*)base + row_bytes + lane_bytes);
+}
+template <typename T>
+__device__ __forceinline__ void st_at(T* base, size_t row_bytes, uint32_t lane_bytes, T v) {
+  *(T*)((char*)base + row_bytes + lane_bytes) = v;
+}
+template <typename T>
+__device__ __forceinline__ void load_row(Raw<T>& r, const T* iCr, const T* iNb, const T* ipd,
+                                         const T* ipl, size_t row_bytes, uint32_t lane_bytes) {
+  r.cr = ld_at(iCr, row_bytes, lane_bytes);
+  r.nb = ld_at(iNb, row_bytes, lane_bytes);
+  r.pd = ld_at(ipd, row_bytes, lane_bytes);
+  r.pl = ld_at(ipl, row_bytes, lane_bytes);
+}
+template <typename T>
+__device__ __forceinline__ void store_row(T* oCr, T* oNb, T* opd, T* opl, size_t row_bytes,
+                                          uint32_t lane_bytes, double a, double b, double c,
+                                          double d) {
+  st_at(oCr, row_bytes, lane_bytes, (T)a);
+  st_at(oNb, row_bytes, lane_bytes, (T)b);
+  st_at(opd, row_bytes, lane_bytes, (T)c);
+  st_at(opl, row_bytes, lane_bytes, (T)d);
+}
+
+// 1/d to within an ulp or so: hardware reciprocal and two Newton steps (the
+// refinement an IEEE division does before its final rounding correction).
+__device__ __forceinline__ double recip(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
+// interpolator p(x) = x^3 (6x^2 - 15x + 10) in Horner form
+__device__ __forceinline__ double p_interp(double x) {
+  return (x * x) * x * fma(x, fma(6.0, x, -15.0), 10.0);
+}
+
+// KKS solve of a cell (src/parabola625.cu:245-297 restated): with
+// p_gamma = 1 - p_delta - p_Laves eliminated, M = A_g + p_d (A_d - A_g) +
+// p_l (A_l - A_g) and r = x - xe_g - p_d (xe_d - xe_g) - p_l (xe_l - xe_g);
+// mu = M^-1 r and the gamma fictitious composition x_g = xe_g + A_g mu.
+constexpr double DA_DEL[3] = {A[1][0] - A[0][0], A[1][1] - A[0][1], A[1][2] - A[0][2]};
+constexpr double DA_LAV[3] = {A[2][0] - A[0][0], A[2][1] - A[0][1], A[2][2] - A[0][2]};
+constexpr double DXE_DEL[2] = {XE[1][0] - XE[0][0], XE[1][1] - XE[0][1]};
+constexpr double DXE_LAV[2] = {XE[2][0] - XE[0][0], XE[2][1] - XE[0][1]};
+
 template <typename T>
 __device__ __forceinline__ void derive(const Raw<T>& r, Point& v, Centre& c) {
   v.pd = (double)r.pd;
   v.pl = (double)r.pl;
   c.xCr = (double)r.cr;
   c.xNb = (double)r.nb;
-  const double pD = p(v.pd);
-  const double pL = p(v.pl);
-  const double pG = 1.0 - pD - pL;
-  kks_mu(c.xCr, c.xNb, pD, pG, pL, c.mu0, c.mu1);
-  v.gCr = fict(0, 0, c.mu0, c.mu1);
-  v.gNb = fict(0, 1, c.mu0, c.mu1);
+  const double pD = p_interp(v.pd);
+  const double pL = p_interp(v.pl);
+  const double M0 = fma(pD, DA_DEL[0], fma(pL, DA_LAV[0], A[0][0]));
+  const double M1 = fma(pD, DA_DEL[1], fma(pL, DA_LAV[1], A[0][1]));
+  const double M2 = fma(pD, DA_DEL[2], fma(pL, DA_LAV[2], A[0][2]));
+  const double r0 = fma(-pD, DXE_DEL[0], fma(-pL, DXE_LAV[0], c.xCr - XE[0][0]));
+  const double r1 = fma(-pD, DXE_DEL[1], fma(-pL, DXE_LAV[1], c.xNb - XE[0][1]));
+  const double inv = recip(fma(M0, M2, -(M1 * M1)));
+  c.mu0 = fma(M2, r0, -(M1 * r1)) * inv;
+  c.mu1 = fma(M0, r1, -(M1 * r0)) * inv;
+  v.gCr = fma(A[0][0], c.mu0, fma(A[0][1], c.mu1, XE[0][0]));
+  v.gNb = fma(A[0][1], c.mu0, fma(A[0][2], c.mu1, XE[0][1]));
 }
 
 __device__ __forceinline__ Point shift_left(const Point& a) {
@@ -181,24 +241,31 @@ __device__ __forceinline__ void laplacians(const double* m, const Point& S, cons
   }
 }
 
+// grand-potential difference gamma - a at potential mu, Horner form of
+// mu.B_a.mu - mu.e_a (B_a = (A_a - A_gamma)/2, e_a = xe_gamma - xe_a)
+__device__ __forceinline__ double grand_potential_diff(const double (&B)[3], const double (&E)[2],
+                                                       double mu0, double mu1) {
+  return fma(mu1, fma(B[2], mu1, -E[1]), mu0 * fma(B[0], mu0, fma(2.0 * B[1], mu1, -E[0])));
+}
+
 // The explicit Euler update of one cell from its stencil Laplacians:
 // Cahn-Hilliard (src/discretization.cu:859-863) and Allen-Cahn
 // (src/discretization.cu:888-942) with the closed-form grand-potential
-// difference at the solved potential.
+// difference at the solved potential; the coefficients carry dt, L and the
+// constant prefactors (StepParams).
 __device__ __forceinline__ void update_cell(const StepParams& P, const double lap[4],
                                             const Point& cur, const Centre& cc, double& nCr,
                                             double& nNb, double& npd, double& npl) {
-  nCr = cc.xCr + P.dt * (P.DCr0 * lap[2] + P.DCr1 * lap[3]);
-  nNb = cc.xNb + P.dt * (P.DNb0 * lap[2] + P.DNb1 * lap[3]);
+  nCr = fma(P.cCr0, lap[2], fma(P.cCr1, lap[3], cc.xCr));
+  nNb = fma(P.cNb0, lap[2], fma(P.cNb1, lap[3], cc.xNb));
   const double fd = cur.pd, fl = cur.pl;
-  const double dOd = dOmega_del(cc.mu0, cc.mu1);
-  const double dOl = dOmega_lav(cc.mu0, cc.mu1);
-  const double dFd = -pPrime(fd) * dOd + 2.0 * P.w0 * fd * (fd - 1.0) * (2.0 * fd - 1.0) +
-                     2.0 * P.alpha * fd * fl * fl - P.k0 * lap[0];
-  const double dFl = -pPrime(fl) * dOl + 2.0 * P.w1 * fl * (fl - 1.0) * (2.0 * fl - 1.0) +
-                     2.0 * P.alpha * fl * fd * fd - P.k1 * lap[1];
-  npd = fd - P.dt * P.L0 * dFd;
-  npl = fl - P.dt * P.L1 * dFl;
+  const double dOd = grand_potential_diff(B_DEL, E_DEL, cc.mu0, cc.mu1);
+  const double dOl = grand_potential_diff(B_LAV, E_LAV, cc.mu0, cc.mu1);
+  const double ud = fma(-fd, fd, fd), ul = fma(-fl, fl, fl);  // phi (1 - phi)
+  const double td = ud * fma(P.a[0] * ud, dOd, P.w[0] * fma(2.0, fd, -1.0));
+  const double tl = ul * fma(P.a[1] * ul, dOl, P.w[1] * fma(2.0, fl, -1.0));
+  npd = fma(-P.c[0] * fd, fl * fl, fma(P.k[0], lap[0], fd) + td);
+  npl = fma(-P.c[1] * fl, fd * fd, fma(P.k[1], lap[1], fl) + tl);
 }
 
 template <typename T>
@@ -261,15 +328,11 @@ __device__ __forceinline__ void march(
   const int y1 = min(y0 + rows, Ny);
   if (y0 >= Ny) return;
 
-  auto row_base = [&](int y) -> size_t {
-    return (size_t)(1 + bc_map<BC>(y, Ny)) * (size_t)pitch + (size_t)px;
-  };
+  const uint32_t in_lane = (uint32_t)px * (uint32_t)sizeof(T);
+  const uint32_t out_lane_b = (uint32_t)(1 + gx) * (uint32_t)sizeof(T);
+  const size_t row_b = (size_t)pitch * sizeof(T);
   auto load = [&](Raw<T>& r, int y) {
-    const size_t a = row_base(y);
-    r.cr = iCr[a];
-    r.nb = iNb[a];
-    r.pd = ipd[a];
-    r.pl = ipl[a];
+    load_row(r, iCr, iNb, ipd, ipl, (size_t)(1 + bc_map<BC>(y, Ny)) * row_b, in_lane);
   };
 
   // window slots (row y0 + k - 1 + 3j lives in slot k % 3 pattern below)
@@ -333,8 +396,7 @@ __device__ __forceinline__ void march(
       }
 
       if (out_lane) {
-        const int qx = 1 + gx;
-        store4(oCr, oNb, opd, opl, (size_t)py * pitch + qx, nCr, nNb, npd, npl);
+        store_row(oCr, oNb, opd, opl, (size_t)py * row_b, out_lane_b, nCr, nNb, npd, npl);
         store_ghost_images<T, BC>(oCr, oNb, opd, opl, gx, y, Nx, Ny, pitch, nCr, nNb, npd, npl);
       }
     }
@@ -387,12 +449,11 @@ __device__ __forceinline__ void march2(
   const int y1 = min(y0 + rows, Ny);
   if (y0 >= Ny) return;
 
+  const uint32_t in_lane = (uint32_t)px * (uint32_t)sizeof(T);
+  const uint32_t out_lane_b = (uint32_t)(1 + gx) * (uint32_t)sizeof(T);
+  const size_t row_b = (size_t)pitch * sizeof(T);
   auto load = [&](Raw<T>& r, int y) {
-    const size_t a = (size_t)(1 + bc_map<BC>(y, Ny)) * (size_t)pitch + (size_t)px;
-    r.cr = iCr[a];
-    r.nb = iNb[a];
-    r.pd = ipd[a];
-    r.pl = ipl[a];
+    load_row(r, iCr, iNb, ipd, ipl, (size_t)(1 + bc_map<BC>(y, Ny)) * row_b, in_lane);
   };
   const double* m = P.m;
 
@@ -473,7 +534,7 @@ __device__ __forceinline__ void march2(
       update_cell(P, lap, b[b1], bcn[b1], nCr, nNb, npd, npl);
 
       if (out_lane) {
-        store4(oCr, oNb, opd, opl, (size_t)(1 + y) * pitch + 1 + gx, nCr, nNb, npd, npl);
+        store_row(oCr, oNb, opd, opl, (size_t)(1 + y) * row_b, out_lane_b, nCr, nNb, npd, npl);
         store_ghost_images<T, BC>(oCr, oNb, opd, opl, gx, y, Nx, Ny, pitch, nCr, nNb, npd, npl);
       }
     }

@@ -23,7 +23,7 @@ __global__ __launch_bounds__(WAVE * WPB) void step_kernel(
   const int nwaves = strips * chunks;
   const int nblocks = (nwaves + WPB - 1) / WPB;
   const int b = (remap ? xcd_remap(blockIdx.x, nblocks) : (int)blockIdx.x) * WPB +
-                (int)(threadIdx.x / WAVE);
+                __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));  // wave-uniform
   if (b >= nwaves) return;  // whole wave exits; no barriers in this kernel
   march<T, BC, NINE, NUC, DEPTH>(iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny, pitch,
                                  b % strips, b / strips, rows, P, N, S);
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(WAVE * WPB) void step2_kernel(
   const int nwaves = strips * chunks;
   const int nblocks = (nwaves + WPB - 1) / WPB;
   const int b = (remap ? xcd_remap(blockIdx.x, nblocks) : (int)blockIdx.x) * WPB +
-                (int)(threadIdx.x / WAVE);
+                __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));  // wave-uniform
   if (b >= nwaves) return;
   march2<T, BC, DEPTH>(iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny, pitch, b % strips,
                        b / strips, rows, P);
