@@ -5,21 +5,26 @@ dx = 0.25 nm, periodic, seeded uniform matrix composition in
 Ni-(0.0202..0.0269)Nb-(0.2794..0.3288)Cr with a central enriched strip (1/8 of
 the width) at Ni-0.169Nb-0.272Cr, nucleation on (Philox seed 42, every
 floor(1e-4/dt) steps, fused into the step), dt = 0.1 dx^2 / (4 D_max).
-A "step" is one fused launch over the whole grid.  value = interior cells x
-timed steps / time of the timed loop (HIP events), fields resident in HBM.
-roofline: algorithmic bytes of the step kernel (64 B per cell-update in FP64:
-4 fields read + 4 written) over its average launch duration (HIP events on
-the launch stream); traffic: HBM bytes per launch from the rocprofv3
-FETCH_SIZE / WRITE_SIZE runs under profiles/ (scaled by a copy kernel).
+A "step" is one explicit Euler step over the whole grid; steps run two per
+launch (the temporally blocked kernel) except nucleation steps, which run one
+per launch with the nucleation pass fused.  value = interior cells x timed
+steps / time of the timed loop (HIP events), fields resident in HBM.
+roofline: the dominant kernel is the two-step kernel; its algorithmic bytes
+per launch (FP64: 4 fields read + 4 written = 64 B per cell, for two
+cell-updates) over its average launch duration (HIP events on the launch
+stream); the one-step kernel is reported beside it.  traffic: HBM bytes per
+launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs under profiles/
+(scaled by a copy kernel).
 cpu_baseline: the FP64 CPU oracle (OpenMP, all host threads) on a bounded
 sample of the same workload.
 
 Also reported (same JSON line):
   batched  -- 512 independent simulations, square sides log-uniform in
               128..1024, FP32 storage with FP64 arithmetic and reductions, one
-              persistent launch per step replayed from a hipGraph, tile and
-              steps-per-graph from the autotuner; against a loop of plain FP64
-              launches over the same instances; phase-fraction error against
+              persistent launch per one or two steps replayed from a hipGraph;
+              tile, steps per launch and launches per graph from the
+              autotuner; against loops of plain FP64 launches (two-step and
+              one-step) over the same instances; phase-fraction error against
               FP64 replays of sampled instances.
   serving  -- Poisson sweep requests (side log-uniform 128..2048, 500..20000
               steps, random composition and nucleation seed) with continuous
@@ -114,37 +119,54 @@ def plain_section(args, dtype_name="f64"):
     cells = N * N
     value = cells * args.steps / (ms / 1e3)
 
-    # the dominant kernel alone: back-to-back step launches (no nucleation steps)
+    # The dominant kernel alone: back-to-back launches of the two-step kernel
+    # (every step pair without a nucleation step), then of the one-step kernel
+    # (nucleation steps; the drop-in API's device_evolution), on the sim's stream.
     P = sim.P
     a, b = sim.a, sim.b
     st = ctypes.c_void_p(stream.cuda_stream)
-    fn = lib().pf_step_f64 if dtype == torch.float64 else lib().pf_step_f32
+    f64 = dtype == torch.float64
     reps = max(20, min(200, args.steps))
-    k0 = torch.cuda.Event(enable_timing=True)
-    k1 = torch.cuda.Event(enable_timing=True)
-    k0.record(stream)
-    for i in range(reps):
-        x, y = (a, b) if i % 2 == 0 else (b, a)
-        check(fn(sim.ctx, s.nx, s.ny, s.bc, ctypes.byref(P), *[_ptr(t) for t in x],
-                 *[_ptr(t) for t in y], None, st), "step")
-    k1.record(stream)
-    torch.cuda.synchronize()
-    kern_s = k0.elapsed_time(k1) / 1e3 / reps
-    bpc = 64 if dtype == torch.float64 else 32
-    alg_bytes = bpc * cells
-    achieved = alg_bytes / kern_s
-    workload = ("plain_fp64_%d" if dtype == torch.float64 else "precision_fp32_%d") % N
+
+    def time_kernel(fn, nuc_arg):
+        k0 = torch.cuda.Event(enable_timing=True)
+        k1 = torch.cuda.Event(enable_timing=True)
+        k0.record(stream)
+        for i in range(reps):
+            x, y = (a, b) if i % 2 == 0 else (b, a)
+            args_ = [sim.ctx, s.nx, s.ny, s.bc, ctypes.byref(P), *[_ptr(t) for t in x],
+                     *[_ptr(t) for t in y]] + ([None] if nuc_arg else []) + [st]
+            check(fn(*args_), "step")
+        k1.record(stream)
+        torch.cuda.synchronize()
+        return k0.elapsed_time(k1) / 1e3 / reps
+
+    kern2_s = time_kernel(lib().pf_step2_f64 if f64 else lib().pf_step2_f32, False)
+    kern1_s = time_kernel(lib().pf_step_f64 if f64 else lib().pf_step_f32, True)
+    bpc = 64 if f64 else 32
+    alg_bytes = bpc * cells          # 4 fields read + 4 written per launch
+    workload = ("plain2_fp64_%d" if f64 else "precision2_fp32_%d") % N
     traffic = load_traffic(workload)
+    traffic1 = load_traffic(("plain_fp64_%d" if f64 else "precision_fp32_%d") % N)
     nuc_every = sim.nuc_every
     sim.close()
     return {
         "value": value, "ms_per_step": ms / args.steps, "N": N, "nuc_every": nuc_every,
         "dt": s.dt,
         "roofline": {
-            "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": traffic,
-            "kernel": "pf::step_kernel (fused step)", "kernel_avg_us": kern_s * 1e6,
+            "bound": "hbm", "achieved": alg_bytes / kern2_s / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": alg_bytes / kern2_s / HBM_PEAK, "traffic": traffic,
+            "kernel": "pf::step2_kernel (two fused steps per launch)",
+            "kernel_avg_us": kern2_s * 1e6,
             "algorithmic_bytes_per_launch": alg_bytes,
+            "cell_updates_per_launch": 2 * cells,
+            "algorithmic_bytes_per_cell_update": bpc // 2,
+            "one_step_kernel": {
+                "kernel": "pf::step_kernel (one fused step per launch)",
+                "kernel_avg_us": kern1_s * 1e6, "achieved": alg_bytes / kern1_s / 1e9,
+                "frac": alg_bytes / kern1_s / HBM_PEAK, "traffic": traffic1,
+                "algorithmic_bytes_per_launch": alg_bytes,
+            },
         },
     }
 
@@ -156,15 +178,15 @@ def batched_section(args):
     from pfpa._lib import PfStepItem, check, lib
     from pfpa.batch import PF_STORAGE_F32, Batch, job_bytes, make_desc
     from pfpa.sim import Simulation
-    from pfpa.workloads import autotune_batch, batch_specs, spec_fields, spec_setup
+    from pfpa.workloads import (autotune_batch, batch_cell_updates, batch_specs, spec_fields,
+                                spec_setup, step_counts)
 
     specs = batch_specs(args.batch_instances, seed=7)
     cells = sum(sp["n"] ** 2 for sp in specs)
-    tune_steps = 8
     pool = sum(job_bytes(PF_STORAGE_F32, sp["n"], sp["n"]) for sp in specs) + (64 << 20)
     batch = Batch(PF_STORAGE_F32, pool_bytes=pool, max_jobs=len(specs))
     total_steps = args.batch_steps
-    target = total_steps + 40 * tune_steps
+    target = total_steps + 1000      # room for the autotuner's launches
     plain_inputs = []
     t_gen = time.perf_counter()
     for sp in specs:
@@ -175,28 +197,33 @@ def batched_section(args):
         plain_inputs.append((s, f))
     log("batched: %d instances, %.3g cells, setup %.1f s" % (len(specs), cells,
                                                             time.perf_counter() - t_gen))
-    best, table = autotune_batch(batch, steps=tune_steps)
-    tuned = {"rows": best[1], "depth": best[2], "steps_per_graph": best[3],
-             "candidates": len(table)}
-    spg = best[3]
-    batch.run(spg, spg)
+    best, table = autotune_batch(batch, launches=8)
+    _, rows, depth, ls, g = best
+    tuned = {"rows": rows, "depth": depth, "launch_steps": ls, "launches_per_graph": g,
+             "candidates": len(table),
+             "table": [{"rate": r[0], "rows": r[1], "depth": r[2], "launch_steps": r[3],
+                        "launches_per_graph": r[4]} for r in sorted(table, reverse=True)]}
+    log("batched autotune:", json.dumps(tuned["table"][:6]))
+    batch.run(g, g)
     batch.sync()
-    steps_before = batch.poll(0)[0]
+    before = step_counts(batch)
+    launches = max(g, (total_steps // ls) // g * g)
     t0 = time.perf_counter()
-    batch.run(total_steps, spg)
+    batch.run(launches, g)
     batch.sync()
     el = time.perf_counter() - t0
-    batched_rate = cells * total_steps / el
-    steps_after = batch.poll(0)[0]
-    assert steps_after - steps_before == total_steps, (steps_before, steps_after)
-    # phase fractions of sampled jobs after steps_after steps
+    done = batch_cell_updates(batch, before)
+    batched_rate = done / el
+    after = step_counts(batch)
+    # phase fractions of sampled jobs after their steps so far
     order = sorted(range(len(specs)), key=lambda k: specs[k]["n"])
     sample = [order[i] for i in (0, len(order) // 3, 2 * len(order) // 3)]
     batched_summ = {k: batch.summarize(k) for k in sample}
     batch.status()
     batch.close()
 
-    # loop of plain FP64 launches (one per instance per step), issued from C
+    # loops of plain FP64 launches over the same instances, issued from C:
+    # two steps per launch (the plain form's fastest) and one step per launch
     ctx = ctypes.c_void_p()
     check(lib().pf_ctx_create(ctypes.byref(ctx)), "ctx")
     bufs = []
@@ -217,16 +244,18 @@ def batched_section(args):
                 it.old_fields[k] = src[k].data_ptr()
                 it.new_fields[k] = dst[k].data_ptr()
     st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    for par in range(2):
-        check(lib().pf_step_loop_f64(ctx, len(bufs), items[par], st), "loop")
-    torch.cuda.synchronize()
-    loop_steps = args.loop_steps
-    t0 = time.perf_counter()
-    for i in range(loop_steps):
-        check(lib().pf_step_loop_f64(ctx, len(bufs), items[i % 2], st), "loop")
-    torch.cuda.synchronize()
-    loop_el = time.perf_counter() - t0
-    loop_rate = cells * loop_steps / loop_el
+    loop_rates = {}
+    for name, fn, per in (("two_step", lib().pf_step2_loop_f64, 2),
+                          ("one_step", lib().pf_step_loop_f64, 1)):
+        for par in range(2):
+            check(fn(ctx, len(bufs), items[par], st), "loop")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.loop_steps):
+            check(fn(ctx, len(bufs), items[i % 2], st), "loop")
+        torch.cuda.synchronize()
+        loop_rates[name] = cells * per * args.loop_steps / (time.perf_counter() - t0)
+    loop_rate = max(loop_rates.values())
     lib().pf_ctx_destroy(ctx)
     del bufs
 
@@ -235,20 +264,21 @@ def batched_section(args):
     for k in sample:
         s, f = plain_inputs[k]
         sim = Simulation(s, f, dtype=torch.float64, nucleation=True, seed=1000 + k, instance=k)
-        sim.step(steps_after)
+        sim.step(after[k])
         ref = sim.summarize()
         sim.close()
         errs.append(max(abs(ref[3] - batched_summ[k][3]), abs(ref[4] - batched_summ[k][4])))
     return {
-        "instances": len(specs), "cells_per_step": cells, "steps": total_steps,
+        "instances": len(specs), "cells_per_step": cells, "launches": launches,
+        "steps_per_job": after[0] - before[0],
         "dtype": "f32-storage/f64-compute",
         "value": batched_rate, "unit": "cell-updates/s",
-        "roofline_frac_f32": batched_rate * 32 / HBM_PEAK,
-        "plain_loop_fp64_value": loop_rate, "plain_loop_steps": loop_steps,
+        "plain_loop_fp64_value": loop_rate, "plain_loop_fp64_rates": loop_rates,
+        "plain_loop_launches": args.loop_steps,
         "speedup_vs_plain_loop": batched_rate / loop_rate,
         "autotuned": tuned,
         "phase_fraction_max_abs_error_vs_fp64": max(errs),
-        "phase_fraction_sample": {"jobs": sample, "steps": steps_after},
+        "phase_fraction_sample": {"jobs": sample, "steps": [after[k] for k in sample]},
     }
 
 
@@ -256,7 +286,8 @@ def serving_section(args):
     from pfpa.serve import fp64_replay_error, make_requests, serve
     reqs = make_requests(args.serve_rate, args.serve_duration, seed=1)
     finished, stats = serve(reqs, pool_bytes=args.serve_pool_gb << 30,
-                            steps_per_graph=args.serve_spg, rows=8, depth=3)
+                            launches_per_graph=args.serve_lpg, rows=16, depth=3,
+                            launch_steps=2)
     sample = sorted(finished, key=lambda r: r.n * r.n * r.steps)[:3]
     err, _ = fp64_replay_error(finished, sample)
     stats["arrival_rate_per_s"] = args.serve_rate
@@ -313,11 +344,13 @@ def main():
     ap.add_argument("--plain-only", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=5000)
-    ap.add_argument("--loop-steps", type=int, default=20)
+    ap.add_argument("--loop-steps", type=int, default=20,
+                    help="launches of each plain loop (two-step and one-step)")
     ap.add_argument("--serve-rate", type=float, default=6.0)
     ap.add_argument("--serve-duration", type=float, default=20.0)
     ap.add_argument("--serve-pool-gb", type=int, default=4)
-    ap.add_argument("--serve-spg", type=int, default=50)
+    ap.add_argument("--serve-lpg", type=int, default=25,
+                    help="step launches per captured graph in the serving loop")
     ap.add_argument("--precision-marks", type=int, nargs="+", default=[1000, 10000, 20000])
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain device copy of 4 x 4098^2 FP64 arrays (1.07 GB moved per "

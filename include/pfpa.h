@@ -157,6 +157,8 @@ typedef struct pf_step_item {
 	double* new_fields[4];
 } pf_step_item;
 int pf_step_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream);
+/* The same loop with pf_step2_f64 (two steps per item and launch). */
+int pf_step2_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream);
 
 /* Heuristic rows-per-block the step kernel uses for an nx x ny grid. */
 int pf_default_rows(int nx, int ny);
@@ -199,6 +201,10 @@ void pf_batch_destroy(pf_batch* batch);
 size_t pf_batch_free_bytes(pf_batch* batch);
 /* Rows per strip task and prefetch depth (2, 3 or 6) of the batched step. */
 int pf_batch_set_tile(pf_batch* batch, int rows, int depth);
+/* Steps per launch: 1, or 2 (the two-step march of pf_step2_*).  With 2 a
+ * launch advances each active job by two steps, or by one where only one
+ * remains or where the job's next step is a nucleation step. */
+int pf_batch_set_launch_steps(pf_batch* batch, int steps);
 /* Admit a job; fields are HOST arrays of the batch's storage type, (Nx+2)*(Ny+2) each.
  * Returns PF_ERR_OVERFLOW when the pool has no room (the caller may retry later). */
 int pf_batch_add(pf_batch* batch, const pf_job_desc* desc, const void* conc_Cr,
@@ -207,9 +213,11 @@ int pf_batch_add(pf_batch* batch, const pf_job_desc* desc, const void* conc_Cr,
 int pf_batch_remove(pf_batch* batch, int job_id);
 /* Publish admissions/removals to the device tables (done by pf_batch_run too). */
 int pf_batch_commit(pf_batch* batch);
-/* Enqueue nsteps steps of every active job; with steps_per_graph > 1 the steps
- * are replayed from a captured hipGraph of that many steps. Asynchronous. */
-int pf_batch_run(pf_batch* batch, int nsteps, int steps_per_graph);
+/* Enqueue `launches` step launches; each advances every active job by its
+ * steps per launch (one, or two: pf_batch_set_launch_steps), and a job stops
+ * at its target.  With launches_per_graph > 1 the launches are replayed from a
+ * captured hipGraph of that many launches.  Asynchronous. */
+int pf_batch_run(pf_batch* batch, int launches, int launches_per_graph);
 int pf_batch_sync(pf_batch* batch);
 /* Synchronous job state. */
 int pf_batch_poll(pf_batch* batch, int job_id, uint64_t* steps_done, int* active);

@@ -1,19 +1,23 @@
 // pf_batch.hip -- many independent simulations of uneven size advanced by one
-// persistent launch per step.
+// persistent launch per step, or per two steps.
 //
 // Layout: one device pool holds every job's two buffer sets (ping-pong), each
 // set four planes (x_Cr, x_Nb, phi_delta, phi_Laves) of pitch x (Ny + 2) cells
 // with the ghost ring, 256-byte aligned.  A device job table describes each job
 // (pool offsets, size, tiling, kinetic parameters, step counter, target) and a
-// wave table maps every strip task (job, strip, row chunk) of the current mix
-// to its job.  The step kernel's grid is fixed (the GPU's resident capacity),
-// so a captured hipGraph of several steps stays valid when jobs are admitted
-// or retired: only the device tables change between replays.
+// wave table maps every strip task (job, 60-column strip, row chunk) of the
+// current mix to its job.  The step kernel's grid is fixed (the GPU's resident
+// capacity), so a captured hipGraph of several launches stays valid when jobs
+// are admitted or retired: only the device tables change between replays.
 //
-// Per step: step kernel (pf::march on every active job's strips, reading the
-// job's set step&1 and writing the other), then the nucleation kernel (jobs
-// whose schedule is due; centre-only embeds), whose last block advances every
-// active job's step counter and retires jobs that reached their target.
+// Per launch: step kernel (on every active job's strips, reading the job's
+// current set and writing the other: pf::march2 when the job advances two
+// steps this launch, pf::march for one), then the nucleation kernel (jobs
+// whose schedule is due on the last step of the launch; centre-only embeds),
+// whose last block advances every active job's step counter, retires jobs
+// that reached their target and decides each job's next advance.  A job
+// advances two steps per launch when two-step launches are on, two steps
+// remain and no nucleation step falls before the second of them.
 // A job's arithmetic and tiling depend only on the job, so its result is the
 // same bits alone or in any mix.
 #include <hip/hip_runtime.h>
@@ -37,7 +41,10 @@ struct BatchJob {
   int strips, chunks, rows, wave0;
   int active;            // 1 while step < target
   int nuc_every;         // 0 = no nucleation
-  int pad[2];
+  int parity;            // buffer set holding the current fields
+  int pair;              // this launch advances two steps (pf::march2)
+  int fuse;              // two-step launches enabled for the batch
+  int pad;
   StepParams P;
   NucParams N;
 };
@@ -78,24 +85,39 @@ __global__ __launch_bounds__(WAVE * WPB) void batch_step_kernel(const BatchJob* 
     if (!J.active) continue;
     const int local = w - J.wave0;
     const int strip = local % J.strips, chunk = local / J.strips;
-    const int par = (int)(J.step & 1);
+    const int par = J.parity;
     const T* in = pool + J.off[par];
     T* out = pool + J.off[par ^ 1];
     const uint64_t pl = J.plane;
-    if (J.bc == 0)
-      march<T, 0, false, false, DEPTH>(in, in + pl, in + 2 * pl, in + 3 * pl, out, out + pl,
-                                       out + 2 * pl, out + 3 * pl, J.Nx, J.Ny, J.pitch, strip,
-                                       chunk, J.rows, J.P, N, S);
-    else
-      march<T, 1, false, false, DEPTH>(in, in + pl, in + 2 * pl, in + 3 * pl, out, out + pl,
-                                       out + 2 * pl, out + 3 * pl, J.Nx, J.Ny, J.pitch, strip,
-                                       chunk, J.rows, J.P, N, S);
+    if (J.pair) {
+      if (J.bc == 0)
+        march2<T, 0, DEPTH>(in, in + pl, in + 2 * pl, in + 3 * pl, out, out + pl, out + 2 * pl,
+                            out + 3 * pl, J.Nx, J.Ny, J.pitch, strip, chunk, J.rows, J.P);
+      else
+        march2<T, 1, DEPTH>(in, in + pl, in + 2 * pl, in + 3 * pl, out, out + pl, out + 2 * pl,
+                            out + 3 * pl, J.Nx, J.Ny, J.pitch, strip, chunk, J.rows, J.P);
+    } else if (J.bc == 0) {
+      march<T, 0, false, false, DEPTH, STRIP2>(in, in + pl, in + 2 * pl, in + 3 * pl, out,
+                                               out + pl, out + 2 * pl, out + 3 * pl, J.Nx, J.Ny,
+                                               J.pitch, strip, chunk, J.rows, J.P, N, S);
+    } else {
+      march<T, 1, false, false, DEPTH, STRIP2>(in, in + pl, in + 2 * pl, in + 3 * pl, out,
+                                               out + pl, out + 2 * pl, out + 3 * pl, J.Nx, J.Ny,
+                                               J.pitch, strip, chunk, J.rows, J.P, N, S);
+    }
   }
 }
 
-// Nucleation on the new fields of the jobs whose schedule is due this step
-// (src/main.cpp:307-316 schedule, centre-only embeds), then the last block
-// advances the step counters.
+// Nucleation on the new fields of the jobs whose schedule is due on the last
+// step of this launch (src/main.cpp:307-316 schedule, centre-only embeds),
+// then the last block advances the step counters and picks each job's next
+// advance (a two-step launch never has a nucleation step before its second
+// step).
+__host__ __device__ __forceinline__ int next_pair(const BatchJob& J) {
+  return J.fuse && J.step + 2 <= J.target &&
+         !(J.nuc_every > 0 && J.step % (uint64_t)J.nuc_every == 0);
+}
+
 template <typename T>
 __global__ __launch_bounds__(WAVE * WPB) void batch_nuc_advance_kernel(
     BatchJob* __restrict__ jobs, const int* __restrict__ wave_job, BatchHeader* __restrict__ hdr,
@@ -106,15 +128,16 @@ __global__ __launch_bounds__(WAVE * WPB) void batch_nuc_advance_kernel(
   for (; w < last; w += stride) {
     const int j = __builtin_amdgcn_readfirstlane(wave_job[w]);
     const BatchJob& J = jobs[j];
-    if (!J.active || J.nuc_every <= 0 || (J.step % (uint64_t)J.nuc_every) != 0) continue;
+    const uint64_t s_last = J.step + (J.pair ? 1 : 0);  // the step whose output this is
+    if (!J.active || J.nuc_every <= 0 || (s_last % (uint64_t)J.nuc_every) != 0) continue;
     const int local = w - J.wave0;
     const int strip = local % J.strips, chunk = local / J.strips;
-    const int gx = strip * STRIP + lane - 1;
-    if (lane < 1 || lane > STRIP || gx >= J.Nx) continue;
-    T* out = pool + J.off[(J.step & 1) ^ 1];
+    const int gx = strip * STRIP2 + lane - 1;
+    if (lane < 1 || lane > STRIP2 || gx >= J.Nx) continue;
+    T* out = pool + J.off[J.parity ^ 1];
     const uint64_t pl = J.plane;
     NucParams N = J.N;
-    N.call = J.step / (uint64_t)J.nuc_every;
+    N.call = s_last / (uint64_t)J.nuc_every;
     const int y0 = chunk * J.rows, y1 = min(y0 + J.rows, J.Ny);
     for (int y = y0; y < y1; y++) {
       const size_t o = (size_t)(1 + y) * J.pitch + 1 + gx;
@@ -149,8 +172,10 @@ __global__ __launch_bounds__(WAVE * WPB) void batch_nuc_advance_kernel(
     for (int j = threadIdx.x; j < hdr->njobs; j += blockDim.x) {
       BatchJob& J = jobs[j];
       if (J.active) {
-        J.step += 1;
+        J.step += J.pair ? 2 : 1;
+        J.parity ^= 1;
         if (J.step >= J.target) J.active = 0;
+        J.pair = next_pair(J);
       }
     }
     __syncthreads();
@@ -184,7 +209,7 @@ __global__ __launch_bounds__(256) void batch_partial_kernel(const BatchJob* __re
   const BatchJob& J = jobs[job];
   const int r = blockIdx.x;
   const int ya = (int)(((long)J.Ny * r) / BATCH_RB), yb = (int)(((long)J.Ny * (r + 1)) / BATCH_RB);
-  const T* f = pool + J.off[J.step & 1];
+  const T* f = pool + J.off[J.parity];
   const uint64_t pl = J.plane;
   KahanB acc[5] = {};
   const long n = (long)(yb - ya) * J.Nx;
@@ -242,6 +267,7 @@ struct pf_batch {
   hipStream_t stream = nullptr;
   int grid_blocks = 0;
   int rows = 16, depth = 3;
+  int fuse = 0;  // two-step launches
   // host mirror
   std::vector<pf::BatchJob> jobs;       // slot -> job (slots are reused)
   std::vector<int> slot_live;           // 1 if the slot holds a job
@@ -316,6 +342,8 @@ int sync_jobs_from_device(pf_batch* b) {
     if (b->slot_live[j] && b->slot_on_device[j]) {
       b->jobs[j].step = dev[j].step;
       b->jobs[j].active = dev[j].active;
+      b->jobs[j].parity = dev[j].parity;
+      b->jobs[j].pair = dev[j].pair;
     }
   return PF_OK;
 }
@@ -373,10 +401,11 @@ int pf_batch_create(pf_batch** out, int storage, size_t pool_bytes, int max_jobs
   b->elem = storage == PF_STORAGE_F32 ? 4 : 8;
   b->pool_elems = pool_bytes / b->elem;
   b->max_jobs = max_jobs;
-  // every wave covers >= 62 x 4 cells of one job; jobs add at most one
+  // every wave covers >= 60 x 4 cells of one job; jobs add at most one
   // partial strip and chunk per dimension
   const size_t cells = b->pool_elems / 8;
-  b->max_waves = (int)std::min<size_t>((size_t)1 << 28, cells / (62 * 4) + (size_t)max_jobs * 64);
+  b->max_waves = (int)std::min<size_t>((size_t)1 << 28,
+                                       cells / (pf::STRIP2 * 4) + (size_t)max_jobs * 64);
   hipError_t e;
   if ((e = hipMalloc(&b->pool, b->pool_elems * b->elem)) != hipSuccess ||
       (e = hipMalloc((void**)&b->d_jobs, (size_t)max_jobs * sizeof(pf::BatchJob))) != hipSuccess ||
@@ -435,6 +464,15 @@ int pf_batch_set_tile(pf_batch* b, int rows, int depth) {
   return PF_OK;
 }
 
+int pf_batch_set_launch_steps(pf_batch* b, int steps) {
+  if (!b) return bfail(PF_ERR_ARG, "null batch");
+  if (steps != 1 && steps != 2) return bfail(PF_ERR_ARG, "steps per launch must be 1 or 2");
+  if (int r = sync_jobs_from_device(b)) return r;
+  b->fuse = steps == 2 ? 1 : 0;
+  b->dirty = true;
+  return PF_OK;
+}
+
 int pf_batch_add(pf_batch* b, const pf_job_desc* d, const void* xCr, const void* xNb,
                  const void* pd, const void* pl, int* job_id) {
   if (!b || !d || !job_id) return bfail(PF_ERR_ARG, "null argument");
@@ -473,6 +511,7 @@ int pf_batch_add(pf_batch* b, const pf_job_desc* d, const void* xCr, const void*
   J.off[0] = off;
   J.off[1] = off + 4 * J.plane;
   J.step = 0;
+  J.parity = 0;
   J.target = d->steps;
   J.Nx = d->Nx;
   J.Ny = d->Ny;
@@ -536,8 +575,10 @@ int pf_batch_commit(pf_batch* b) {
       continue;
     }
     J.rows = b->rows;
-    J.strips = (J.Nx + pf::STRIP - 1) / pf::STRIP;
+    J.strips = (J.Nx + pf::STRIP2 - 1) / pf::STRIP2;
     J.chunks = (J.Ny + J.rows - 1) / J.rows;
+    J.fuse = b->fuse && J.Nx >= 3 && J.Ny >= 3;
+    J.pair = J.active ? pf::next_pair(J) : 0;
     J.wave0 = (int)wave_job.size();
     if (!J.active) {
       J.wave0 = 0;
@@ -569,14 +610,14 @@ int pf_batch_commit(pf_batch* b) {
   return PF_OK;
 }
 
-int pf_batch_run(pf_batch* b, int nsteps, int steps_per_graph) {
+int pf_batch_run(pf_batch* b, int launches, int per_graph) {
   if (!b) return bfail(PF_ERR_ARG, "null batch");
-  if (nsteps < 0 || steps_per_graph < 0) return bfail(PF_ERR_ARG, "negative step count");
+  if (launches < 0 || per_graph < 0) return bfail(PF_ERR_ARG, "negative launch count");
   if (int r = pf_batch_commit(b)) return r;
   hipError_t e;
   int done = 0;
-  if (steps_per_graph > 1) {
-    if (!b->graph || b->graph_steps != steps_per_graph || b->graph_depth != b->depth) {
+  if (per_graph > 1) {
+    if (!b->graph || b->graph_steps != per_graph || b->graph_depth != b->depth) {
       if (b->graph) {
         (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
@@ -585,20 +626,20 @@ int pf_batch_run(pf_batch* b, int nsteps, int steps_per_graph) {
       if ((e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
         return bhip(e, "begin capture");
       hipError_t le = hipSuccess;
-      for (int s = 0; s < steps_per_graph && le == hipSuccess; s++) le = enqueue_step(b, b->stream);
+      for (int s = 0; s < per_graph && le == hipSuccess; s++) le = enqueue_step(b, b->stream);
       e = hipStreamEndCapture(b->stream, &g);
       if (le != hipSuccess) return bhip(le, "capture step");
       if (e != hipSuccess) return bhip(e, "end capture");
       e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       if (e != hipSuccess) return bhip(e, "instantiate graph");
-      b->graph_steps = steps_per_graph;
+      b->graph_steps = per_graph;
       b->graph_depth = b->depth;
     }
-    for (; done + steps_per_graph <= nsteps; done += steps_per_graph)
+    for (; done + per_graph <= launches; done += per_graph)
       if ((e = hipGraphLaunch(b->graph, b->stream)) != hipSuccess) return bhip(e, "graph launch");
   }
-  for (; done < nsteps; done++)
+  for (; done < launches; done++)
     if ((e = enqueue_step(b, b->stream)) != hipSuccess) return bhip(e, "step launch");
   return PF_OK;
 }
@@ -646,7 +687,7 @@ int pf_batch_read(pf_batch* b, int job_id, void* xCr, void* xNb, void* pd, void*
   const pf::BatchJob& J = b->jobs[job_id];
   const size_t bytes = (size_t)(J.Nx + 2) * (J.Ny + 2) * b->elem;
   void* dst[4] = {xCr, xNb, pd, pl};
-  const char* base = (const char*)b->pool + J.off[J.step & 1] * b->elem;
+  const char* base = (const char*)b->pool + J.off[J.parity] * b->elem;
   for (int k = 0; k < 4; k++) {
     if (!dst[k]) continue;
     hipError_t e = hipMemcpy(dst[k], base + k * J.plane * b->elem, bytes, hipMemcpyDeviceToHost);
@@ -696,7 +737,7 @@ int pf_batch_waves(pf_batch* b) {
   int total = 0;
   for (size_t j = 0; j < b->jobs.size(); j++)
     if (b->slot_live[j] && b->jobs[j].active)
-      total += ((b->jobs[j].Nx + pf::STRIP - 1) / pf::STRIP) *
+      total += ((b->jobs[j].Nx + pf::STRIP2 - 1) / pf::STRIP2) *
                ((b->jobs[j].Ny + b->rows - 1) / b->rows);
   return total;
 }

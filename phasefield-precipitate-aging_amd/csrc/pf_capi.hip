@@ -229,6 +229,19 @@ int pf_step_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream
   return PF_OK;
 }
 
+int pf_step2_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream) {
+  if (n < 0 || (n > 0 && !items)) return fail(PF_ERR_ARG, "bad item list");
+  for (int i = 0; i < n; i++) {
+    const pf_step_item& it = items[i];
+    if (int r = step2_t<double>(ctx, it.nx, it.ny, it.bc, it.prm, it.old_fields[0],
+                                it.old_fields[1], it.old_fields[2], it.old_fields[3],
+                                it.new_fields[0], it.new_fields[1], it.new_fields[2],
+                                it.new_fields[3], stream))
+      return r;
+  }
+  return PF_OK;
+}
+
 int pf_nucleation_f64(pf_ctx* ctx, int nx, int ny, double* a, double* b, double* c, double* d,
                       const pf_nuc_params* nuc, void* stream) {
   return nucleation_t<double>(ctx, nx, ny, a, b, c, d, nuc, stream);

@@ -28,7 +28,8 @@
 namespace pf {
 
 constexpr int WAVE = 64;
-constexpr int STRIP = WAVE - 2;  // output columns per strip
+constexpr int STRIP = WAVE - 2;   // output columns per strip
+constexpr int STRIP2 = WAVE - 4;  // output columns per strip of the two-step march
 constexpr int WPB = 4;           // waves (strips) per block: adjacent strips of one row chunk
 constexpr int UNROLL = 6;        // multiple of the 3-row window and of every ring depth
 
@@ -310,8 +311,9 @@ __device__ __forceinline__ void store_ghost_images(T* oCr, T* oNb, T* opd, T* op
 
 // Advance strip `strip` of row chunk `chunk` (rows [chunk*rows, ...)) by one
 // step, reading the old fields and writing the new ones.  Called by a whole
-// wave; uses no barriers.
-template <typename T, int BC, bool NINE, bool NUC, int DEPTH>
+// wave; uses no barriers.  SW is the number of output columns per strip (62,
+// or 60 where the strips are laid out for the two-step march).
+template <typename T, int BC, bool NINE, bool NUC, int DEPTH, int SW = STRIP>
 __device__ __forceinline__ void march(
     const T* __restrict__ iCr, const T* __restrict__ iNb, const T* __restrict__ ipd,
     const T* __restrict__ ipl, T* __restrict__ oCr, T* __restrict__ oNb, T* __restrict__ opd,
@@ -320,9 +322,10 @@ __device__ __forceinline__ void march(
   static_assert(UNROLL % DEPTH == 0, "ring depth must divide the unroll factor");
   const int lane = threadIdx.x & (WAVE - 1);
 
-  const int gx = strip * STRIP + lane - 1;  // interior column owned by this lane
-  const int px = 1 + bc_map<BC>(gx, Nx);    // padded column it reads
-  const bool out_lane = (lane >= 1) && (lane <= STRIP) && (gx < Nx);
+  static_assert(SW >= 1 && SW <= STRIP, "strip width");
+  const int gx = strip * SW + lane - 1;   // interior column owned by this lane
+  const int px = 1 + bc_map<BC>(gx, Nx);  // padded column it reads
+  const bool out_lane = (lane >= 1) && (lane <= SW) && (gx < Nx);
 
   const int y0 = chunk * rows;
   const int y1 = min(y0 + rows, Ny);
@@ -428,7 +431,6 @@ __device__ __forceinline__ void march(
 // stage-A row in y.  So two steps here give the same result as two one-step
 // launches.  Five-point masks, no nucleation (the caller runs nucleation
 // steps one at a time).
-constexpr int STRIP2 = WAVE - 4;  // output columns per two-step strip
 
 template <typename T, int BC, int DEPTH>
 __device__ __forceinline__ void march2(

@@ -70,6 +70,7 @@ PROTOTYPES = {
     "pf_philox_eval": (_i, [_i, _vp, _vp, _vp]),
     "pf_default_rows": (_i, [_i, _i]),
     "pf_step_loop_f64": (_i, [_vp, _i, ctypes.POINTER(PfStepItem), _vp]),
+    "pf_step2_loop_f64": (_i, [_vp, _i, ctypes.POINTER(PfStepItem), _vp]),
     "pf_host_p": (_d, [_d]),
     "pf_host_pPrime": (_d, [_d]),
     "pf_host_g": (_d, [_i, _d, _d]),

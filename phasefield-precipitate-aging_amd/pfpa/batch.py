@@ -29,6 +29,7 @@ PROTOS = {
     "pf_batch_destroy": (None, [_vp]),
     "pf_batch_free_bytes": (ctypes.c_size_t, [_vp]),
     "pf_batch_set_tile": (_i, [_vp, _i, _i]),
+    "pf_batch_set_launch_steps": (_i, [_vp, _i]),
     "pf_batch_add": (_i, [_vp, ctypes.POINTER(PfJobDesc), _vp, _vp, _vp, _vp,
                           ctypes.POINTER(_i)]),
     "pf_batch_remove": (_i, [_vp, _i]),
@@ -95,9 +96,15 @@ class Batch:
               "pf_batch_create")
         self.h = h
         self.shapes = {}
+        self.launch_steps = 1
 
     def set_tile(self, rows, depth=3):
         check(blib().pf_batch_set_tile(self.h, rows, depth), "pf_batch_set_tile")
+
+    def set_launch_steps(self, steps):
+        """1: one step per launch; 2: two (the temporally blocked march)."""
+        check(blib().pf_batch_set_launch_steps(self.h, steps), "pf_batch_set_launch_steps")
+        self.launch_steps = steps
 
     def add(self, desc, fields):
         """Admit a job; returns its id, or None if the pool is full."""
@@ -121,8 +128,12 @@ class Batch:
     def commit(self):
         check(blib().pf_batch_commit(self.h), "pf_batch_commit")
 
-    def run(self, nsteps, steps_per_graph=0):
-        check(blib().pf_batch_run(self.h, int(nsteps), int(steps_per_graph)), "pf_batch_run")
+    def run(self, launches, launches_per_graph=0):
+        """Enqueue `launches` step launches (each advances every active job by
+        one step, or two with set_launch_steps(2)), replayed from a captured
+        graph of `launches_per_graph` launches when that is > 1."""
+        check(blib().pf_batch_run(self.h, int(launches), int(launches_per_graph)),
+              "pf_batch_run")
 
     def sync(self):
         check(blib().pf_batch_sync(self.h), "pf_batch_sync")

@@ -70,11 +70,14 @@ def request_desc(r, dx=0.25e-9):
     return s, make_desc(s, r.steps, nucleation=True, seed=r.seed, instance=r.rid)
 
 
-def serve(requests, pool_bytes, steps_per_graph=50, rows=8, depth=3, log=None,
-          storage=PF_STORAGE_F32, time_limit=None):
-    """Run the request stream; returns (requests, stats)."""
+def serve(requests, pool_bytes, launches_per_graph=25, rows=16, depth=3, launch_steps=2,
+          log=None, storage=PF_STORAGE_F32, time_limit=None):
+    """Run the request stream; returns (requests, stats).  Each device round is
+    one replay of a graph of `launches_per_graph` step launches, each advancing
+    every running job by `launch_steps` steps (1 or 2)."""
     batch = Batch(storage, pool_bytes=pool_bytes, max_jobs=max(16, len(requests)))
     batch.set_tile(rows, depth)
+    batch.set_launch_steps(launch_steps)
     pending = list(requests)       # sorted by arrival
     waiting = []
     running = {}                   # job id -> request
@@ -109,7 +112,7 @@ def serve(requests, pool_bytes, steps_per_graph=50, rows=8, depth=3, log=None,
             running[jid] = r
         waiting = still
         if running:
-            batch.run(steps_per_graph, steps_per_graph)   # asynchronous
+            batch.run(launches_per_graph, launches_per_graph)   # asynchronous
             # build the next admissions' fields while the device works
             now = time.perf_counter() - t0
             for r in waiting[:4] + [q for q in pending[:4] if q.arrival <= now + 0.05]:
@@ -144,7 +147,8 @@ def serve(requests, pool_bytes, steps_per_graph=50, rows=8, depth=3, log=None,
         "latency_p50_s": float(np.percentile(lat, 50)),
         "latency_p99_s": float(np.percentile(lat, 99)),
         "latency_max_s": float(lat.max()),
-        "steps_per_graph": steps_per_graph,
+        "launches_per_graph": launches_per_graph,
+        "launch_steps": launch_steps,
         "pool_bytes": pool_bytes,
     }
     return finished, stats

@@ -45,37 +45,59 @@ def spec_fields(spec, dx=0.25e-9):
                                             radius=radius)
 
 
-def autotune_batch(batch, steps=8, rows_set=(4, 8, 16, 32), depth_set=(2, 3, 6),
-                   graph_set=(1, 8)):
-    """Time `steps` steps of the current job mix for each tile configuration
-    and steps-per-graph; leave the batch on the fastest.  The jobs advance
-    during tuning (their targets must allow it).  Returns (best, table)."""
+def batch_cell_updates(batch, before):
+    """Cell-updates done since `before` (a dict job -> steps from
+    step_counts()), from the jobs' own step counters."""
+    total = 0
+    for j, steps, _ in batch.poll_all():
+        ny, nx = batch.shapes[j]
+        total += (steps - before.get(j, 0)) * (nx - 2) * (ny - 2)
+    return total
+
+
+def step_counts(batch):
+    return {j: steps for j, steps, _ in batch.poll_all()}
+
+
+def autotune_batch(batch, launches=8, rows_set=(8, 16, 32), depth_set=(2, 3, 6),
+                   launch_steps_set=(1, 2), graph_set=(1, 8)):
+    """Measure the current job mix's cell-updates/s for each tile shape
+    (rows, prefetch depth), time steps per launch (one, or two with the
+    temporally blocked march) and launches per captured graph; leave the batch
+    on the fastest.  Rates come from the jobs' step counters, so launches that
+    advance some jobs by one step (nucleation steps, last steps) are counted
+    exactly.  The jobs advance during tuning (their targets must allow it).
+    Returns (best, table), entries (rate, rows, depth, launch_steps, per_graph)."""
     table = []
-    for rows in rows_set:
-        for depth in depth_set:
-            batch.set_tile(rows, depth)
-            batch.commit()
-            batch.run(2, 0)            # warm: compile, caches
-            batch.sync()
-            t0 = time.perf_counter()
-            batch.run(steps, 0)
-            batch.sync()
-            table.append(((time.perf_counter() - t0) / steps, rows, depth, 1))
-    table.sort()
-    _, rows, depth, _ = table[0]
+
+    def measure(n, per_graph):
+        batch.sync()
+        before = step_counts(batch)
+        t0 = time.perf_counter()
+        batch.run(n, per_graph)
+        batch.sync()
+        el = time.perf_counter() - t0
+        return batch_cell_updates(batch, before) / el
+
+    for ls in launch_steps_set:
+        batch.set_launch_steps(ls)
+        for rows in rows_set:
+            for depth in depth_set:
+                batch.set_tile(rows, depth)
+                batch.commit()
+                batch.run(2, 0)            # warm: caches
+                table.append((measure(launches, 0), rows, depth, ls, 1))
+    best = max(table)
+    _, rows, depth, ls, _ = best
+    batch.set_launch_steps(ls)
     batch.set_tile(rows, depth)
     batch.commit()
-    best = table[0]
-    for spg in graph_set:
-        if spg <= 1:
+    for g in graph_set:
+        if g <= 1:
             continue
-        batch.run(spg, spg)            # capture
-        batch.sync()
-        t0 = time.perf_counter()
-        batch.run(spg * max(1, steps // spg), spg)
-        batch.sync()
-        dt = (time.perf_counter() - t0) / (spg * max(1, steps // spg))
-        table.append((dt, rows, depth, spg))
-        if dt < best[0]:
-            best = (dt, rows, depth, spg)
+        batch.run(g, g)                    # capture
+        rate = measure(g * max(1, launches // g), g)
+        table.append((rate, rows, depth, ls, g))
+        if rate > best[0]:
+            best = (rate, rows, depth, ls, g)
     return best, table

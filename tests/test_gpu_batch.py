@@ -44,11 +44,15 @@ def plain(setup, fields, steps, dtype, **kw):
 JOBS = [(64, 64, 1), (130, 71, 0), (37, 200, 1), (200, 33, 0), (1, 5, 1), (63, 62, 0)]
 
 
+@pytest.mark.parametrize("launch_steps", [1, 2])
 @pytest.mark.parametrize("storage", [PF_STORAGE_F64, PF_STORAGE_F32])
-def test_batch_equals_plain_alone(gpu, storage):
+def test_batch_equals_plain_alone(gpu, storage, launch_steps):
+    """Each job (odd and even targets, tiny and ragged grids) against the
+    plain path run alone, with one and with two steps per launch."""
     dtype = torch.float64 if storage == PF_STORAGE_F64 else torch.float32
     b = Batch(storage, pool_bytes=256 << 20)
     b.set_tile(8, 3)
+    b.set_launch_steps(launch_steps)
     ids = []
     setups = []
     for k, (Nx, Ny, bc) in enumerate(JOBS):
@@ -56,46 +60,55 @@ def test_batch_equals_plain_alone(gpu, storage):
         f = rand_fields(Nx, Ny, bc, 100 + k)
         ids.append(b.add(make_desc(s, steps=17 + k, instance=k), f))
         setups.append((s, f, 17 + k))
-    b.run(40)          # more steps than any job's target: each stops at its own
+    b.run(40)          # more launches than any job needs: each stops at its own target
     for jid, (s, f, n) in zip(ids, setups):
         steps, active = b.poll(jid)
         assert steps == n and not active
         got = b.read(jid)
-        want = plain(s, f, n, dtype)
-        for k in range(4):
-            assert np.array_equal(got[k].astype(np.float64), want[k]), (jid, k)
+        for fuse in (False, True):
+            want = plain(s, f, n, dtype, fuse_steps=fuse)
+            for k in range(4):
+                assert np.array_equal(got[k].astype(np.float64), want[k]), (jid, fuse, k)
     b.close()
 
 
-def test_alone_vs_mix_graph_and_eager(gpu):
-    """FP32 job X: alone (eager) == in a mix (graph replays, admissions in
-    between) == the plain FP32 path."""
-    s = M.make_setup(96, 80, bc=1)
-    fx = rand_fields(96, 80, 1, 7)
-    steps = 30
+@pytest.mark.parametrize("launch_steps", [1, 2])
+def test_alone_vs_mix_graph_and_eager(gpu, launch_steps):
+    """FP32 job X with nucleation on its schedule: alone (eager) == in a mix
+    (graph replays, admissions and a removal in between, other jobs at other
+    points of their schedules) == the plain FP32 path."""
+    s = M.make_setup(64, 48, bc=1)
+    fx = nuc_fields(64, 48, 1, 7)
+    steps = 29   # the last step (28) is a nucleation step: embedded cells hold exactly 1
+    desc = dict(nucleation=True, seed=77, instance=5, nuc_every=4, nuc_dx=NUC_DX)
     alone = Batch(PF_STORAGE_F32, pool_bytes=64 << 20)
-    jx = alone.add(make_desc(s, steps, instance=5), fx)
+    alone.set_launch_steps(launch_steps)
+    jx = alone.add(make_desc(s, steps, **desc), fx)
     alone.run(steps)
     a = alone.read(jx)
     alone.close()
 
     mix = Batch(PF_STORAGE_F32, pool_bytes=256 << 20)
     mix.set_tile(4, 6)
+    mix.set_launch_steps(launch_steps)
     others = []
     for k, (Nx, Ny, bc) in enumerate(JOBS):
         so = M.make_setup(Nx, Ny, bc=bc)
         others.append(mix.add(make_desc(so, 1000, instance=50 + k), rand_fields(Nx, Ny, bc, k)))
-    mix.run(7, steps_per_graph=3)          # others advance before X arrives
-    jx2 = mix.add(make_desc(s, steps, instance=5), fx)
-    mix.run(10, steps_per_graph=4)
+    mix.run(7, 3)                          # others advance before X arrives
+    jx2 = mix.add(make_desc(s, steps, **desc), fx)
+    mix.run(5, 4)
     mix.remove(others[1])                  # retire one while X runs
     so = M.make_setup(150, 90, bc=0)
     mix.add(make_desc(so, 1000, instance=99), rand_fields(150, 90, 0, 3))
-    mix.run(steps - 10, steps_per_graph=4)
+    mix.run(steps, 4)
     assert mix.poll(jx2) == (steps, False)
     m = mix.read(jx2)
     mix.close()
-    want = plain(s, fx, steps, torch.float32)
+    want = plain(s, fx, steps, torch.float32, nucleation=True, seed=77, instance=5, nuc_every=4,
+                 nuc_dx=NUC_DX)
+    nucleated = (np.asarray(want[2]) == 1.0) | (np.asarray(want[3]) == 1.0)
+    assert nucleated[1:-1, 1:-1].sum() > 5   # cells embedded by the last pass
     for k in range(4):
         assert np.array_equal(a[k], m[k]), k
         assert np.array_equal(a[k].astype(np.float64), want[k]), k
@@ -105,28 +118,34 @@ def test_graph_replays_match_eager(gpu):
     s = M.make_setup(120, 70, bc=0)
     f = rand_fields(120, 70, 0, 11)
     res = []
-    for spg in (0, 4):
+    for spg, ls in ((0, 1), (4, 1), (0, 2), (4, 2)):
         b = Batch(PF_STORAGE_F64, pool_bytes=64 << 20)
-        j = b.add(make_desc(s, 100), f)
+        b.set_launch_steps(ls)
+        j = b.add(make_desc(s, 80), f)
         for r in range(5):                 # eager host work between replays
-            b.run(8, steps_per_graph=spg)
+            b.run(8 * (3 - ls), spg)
             b.poll(j)
+        assert b.poll(j) == (80, False)
         res.append(b.read(j))
         b.close()
-    for k in range(4):
-        assert np.array_equal(res[0][k], res[1][k])
+    for r in res[1:]:
+        for k in range(4):
+            assert np.array_equal(res[0][k], r[k])
 
 
-def test_batch_nucleation_matches_plain(gpu):
+@pytest.mark.parametrize("launch_steps", [1, 2])
+def test_batch_nucleation_matches_plain(gpu, launch_steps):
     """Nucleation schedule, random stream and embeds: identical to the plain
     fused path, with P inside (0, 1)."""
     for storage, dtype in ((PF_STORAGE_F64, torch.float64), (PF_STORAGE_F32, torch.float32)):
         s = M.make_setup(64, 48, bc=1)
         f = nuc_fields(64, 48, 1, 3)
         b = Batch(storage, pool_bytes=64 << 20)
+        b.set_launch_steps(launch_steps)
         j = b.add(make_desc(s, 7, nucleation=True, seed=123, instance=4, nuc_every=3,
                             nuc_dx=NUC_DX), f)
         b.run(7)
+        assert b.poll(j) == (7, False)
         got = b.read(j)
         assert b.status() == 0
         b.close()
