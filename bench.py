@@ -5,27 +5,27 @@ dx = 0.25 nm, periodic, seeded uniform matrix composition in
 Ni-(0.0202..0.0269)Nb-(0.2794..0.3288)Cr with a central enriched strip (1/8 of
 the width) at Ni-0.169Nb-0.272Cr, nucleation on (Philox seed 42, every
 floor(1e-4/dt) steps, fused into the step), dt = 0.1 dx^2 / (4 D_max).
-A "step" is one explicit Euler step over the whole grid; steps run two per
-launch (the temporally blocked kernel) except nucleation steps, which run one
-per launch with the nucleation pass fused.  value = interior cells x timed
-steps / time of the timed loop (HIP events), fields resident in HBM.
-roofline: the dominant kernel is the two-step kernel; its algorithmic bytes
-per launch (FP64: 4 fields read + 4 written = 64 B per cell, for two
+A "step" is one explicit Euler step over the whole grid; steps run three per
+launch (the temporally blocked kernel) except around nucleation steps, which
+run one per launch with the nucleation pass fused.  value = interior cells x
+timed steps / time of the timed loop (HIP events), fields resident in HBM.
+roofline: the dominant kernel is the three-step kernel; its algorithmic bytes
+per launch (FP64: 4 fields read + 4 written = 64 B per cell, for three
 cell-updates) over its average launch duration (HIP events on the launch
-stream); the one-step kernel is reported beside it.  traffic: HBM bytes per
-launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs under profiles/
-(scaled by a copy kernel).
+stream); the two-step and one-step kernels are reported beside it.  traffic:
+HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs under
+profiles/ (scaled by a copy kernel).
 cpu_baseline: the FP64 CPU oracle (OpenMP, all host threads) on a bounded
 sample of the same workload.
 
 Also reported (same JSON line):
   batched  -- 512 independent simulations, square sides log-uniform in
               128..1024, FP32 storage with FP64 arithmetic and reductions, one
-              persistent launch per one or two steps replayed from a hipGraph;
-              tile, steps per launch and launches per graph from the
-              autotuner; against loops of plain FP64 launches (two-step and
-              one-step) over the same instances; phase-fraction error against
-              FP64 replays of sampled instances.
+              persistent launch per one, two or three steps replayed from a
+              hipGraph; tile, steps per launch and launches per graph from the
+              autotuner; against loops of plain FP64 launches (three-, two-
+              and one-step) over the same instances; phase-fraction error
+              against FP64 replays of sampled instances.
   serving  -- Poisson sweep requests (side log-uniform 128..2048, 500..20000
               steps, random composition and nucleation seed) with continuous
               batching under an HBM budget; throughput and p50/p99 latency.
@@ -141,32 +141,36 @@ def plain_section(args, dtype_name="f64"):
         torch.cuda.synchronize()
         return k0.elapsed_time(k1) / 1e3 / reps
 
+    kern3_s = time_kernel(lib().pf_step3_f64 if f64 else lib().pf_step3_f32, False)
     kern2_s = time_kernel(lib().pf_step2_f64 if f64 else lib().pf_step2_f32, False)
     kern1_s = time_kernel(lib().pf_step_f64 if f64 else lib().pf_step_f32, True)
     bpc = 64 if f64 else 32
     alg_bytes = bpc * cells          # 4 fields read + 4 written per launch
-    workload = ("plain2_fp64_%d" if f64 else "precision2_fp32_%d") % N
-    traffic = load_traffic(workload)
-    traffic1 = load_traffic(("plain_fp64_%d" if f64 else "precision_fp32_%d") % N)
+    tag = "fp64" if f64 else "fp32"
+
+    def other(name, k, secs):
+        return {"kernel": name, "steps_per_launch": k, "kernel_avg_us": secs * 1e6,
+                "achieved": alg_bytes / secs / 1e9, "frac": alg_bytes / secs / HBM_PEAK,
+                "traffic": load_traffic("plain%s_%s_%d" % ("" if k == 1 else k, tag, N)),
+                "algorithmic_bytes_per_launch": alg_bytes}
+
     nuc_every = sim.nuc_every
     sim.close()
     return {
         "value": value, "ms_per_step": ms / args.steps, "N": N, "nuc_every": nuc_every,
         "dt": s.dt,
         "roofline": {
-            "bound": "hbm", "achieved": alg_bytes / kern2_s / 1e9, "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": alg_bytes / kern2_s / HBM_PEAK, "traffic": traffic,
-            "kernel": "pf::step2_kernel (two fused steps per launch)",
-            "kernel_avg_us": kern2_s * 1e6,
+            "bound": "hbm", "achieved": alg_bytes / kern3_s / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": alg_bytes / kern3_s / HBM_PEAK,
+            "traffic": load_traffic("plain3_%s_%d" % (tag, N)),
+            "kernel": "pf::stepk_kernel<K=3> (three fused steps per launch)",
+            "kernel_avg_us": kern3_s * 1e6,
             "algorithmic_bytes_per_launch": alg_bytes,
-            "cell_updates_per_launch": 2 * cells,
-            "algorithmic_bytes_per_cell_update": bpc // 2,
-            "one_step_kernel": {
-                "kernel": "pf::step_kernel (one fused step per launch)",
-                "kernel_avg_us": kern1_s * 1e6, "achieved": alg_bytes / kern1_s / 1e9,
-                "frac": alg_bytes / kern1_s / HBM_PEAK, "traffic": traffic1,
-                "algorithmic_bytes_per_launch": alg_bytes,
-            },
+            "cell_updates_per_launch": 3 * cells,
+            "algorithmic_bytes_per_cell_update": bpc / 3,
+            "two_step_kernel": other("pf::stepk_kernel<K=2> (two fused steps per launch)", 2,
+                                     kern2_s),
+            "one_step_kernel": other("pf::step_kernel (one fused step per launch)", 1, kern1_s),
         },
     }
 
@@ -222,8 +226,8 @@ def batched_section(args):
     batch.status()
     batch.close()
 
-    # loops of plain FP64 launches over the same instances, issued from C:
-    # two steps per launch (the plain form's fastest) and one step per launch
+    # loops of plain FP64 launches over the same instances, issued from C, at
+    # three, two and one steps per launch; the fastest is the comparison
     ctx = ctypes.c_void_p()
     check(lib().pf_ctx_create(ctypes.byref(ctx)), "ctx")
     bufs = []
@@ -245,7 +249,8 @@ def batched_section(args):
                 it.new_fields[k] = dst[k].data_ptr()
     st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     loop_rates = {}
-    for name, fn, per in (("two_step", lib().pf_step2_loop_f64, 2),
+    for name, fn, per in (("three_step", lib().pf_step3_loop_f64, 3),
+                          ("two_step", lib().pf_step2_loop_f64, 2),
                           ("one_step", lib().pf_step_loop_f64, 1)):
         for par in range(2):
             check(fn(ctx, len(bufs), items[par], st), "loop")
@@ -287,7 +292,7 @@ def serving_section(args):
     reqs = make_requests(args.serve_rate, args.serve_duration, seed=1)
     finished, stats = serve(reqs, pool_bytes=args.serve_pool_gb << 30,
                             launches_per_graph=args.serve_lpg, rows=16, depth=3,
-                            launch_steps=2)
+                            launch_steps=3)
     sample = sorted(finished, key=lambda r: r.n * r.n * r.steps)[:3]
     err, _ = fp64_replay_error(finished, sample)
     stats["arrival_rate_per_s"] = args.serve_rate
@@ -349,7 +354,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=6.0)
     ap.add_argument("--serve-duration", type=float, default=20.0)
     ap.add_argument("--serve-pool-gb", type=int, default=4)
-    ap.add_argument("--serve-lpg", type=int, default=25,
+    ap.add_argument("--serve-lpg", type=int, default=17,
                     help="step launches per captured graph in the serving loop")
     ap.add_argument("--precision-marks", type=int, nargs="+", default=[1000, 10000, 20000])
     ap.add_argument("--calibrate-copy", action="store_true",

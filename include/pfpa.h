@@ -109,6 +109,19 @@ int pf_step2_f32(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm,
                  const float* phi_del_old, const float* phi_lav_old,
                  float* conc_Cr_new, float* conc_Nb_new,
                  float* phi_del_new, float* phi_lav_new, void* stream);
+/* Three fused steps, old -> new: the same as three pf_step_* calls without
+ * nucleation (src/main.cpp:289-305, 318-321 run three times).  Same
+ * restrictions as pf_step2_*. */
+int pf_step3_f64(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm,
+                 const double* conc_Cr_old, const double* conc_Nb_old,
+                 const double* phi_del_old, const double* phi_lav_old,
+                 double* conc_Cr_new, double* conc_Nb_new,
+                 double* phi_del_new, double* phi_lav_new, void* stream);
+int pf_step3_f32(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm,
+                 const float* conc_Cr_old, const float* conc_Nb_old,
+                 const float* phi_del_old, const float* phi_lav_old,
+                 float* conc_Cr_new, float* conc_Nb_new,
+                 float* phi_del_new, float* phi_lav_new, void* stream);
 
 /* Standalone nucleation pass in place on the given fields.  Replaces
  * device_nucleation (src/discretization.cu:1179-1209). */
@@ -157,16 +170,20 @@ typedef struct pf_step_item {
 	double* new_fields[4];
 } pf_step_item;
 int pf_step_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream);
-/* The same loop with pf_step2_f64 (two steps per item and launch). */
+/* The same loop with pf_step2_f64 / pf_step3_f64 (two / three steps per item
+ * and launch). */
 int pf_step2_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream);
+int pf_step3_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream);
 
 /* Heuristic rows-per-block the step kernel uses for an nx x ny grid. */
 int pf_default_rows(int nx, int ny);
-/* Tile of the two-step kernel: rows per chunk (0 = heuristic) and prefetch
- * depth (2, 3, 6; 0 = 3).  pf_default_rows2 is its heuristic for storage
- * PF_STORAGE_F64 or PF_STORAGE_F32. */
+/* Tile of the two-step (three-step) kernel: rows per chunk (0 = heuristic)
+ * and prefetch depth (2, 3, 6; 0 = 3).  pf_default_rows2 (3) is its heuristic
+ * for storage PF_STORAGE_F64 or PF_STORAGE_F32. */
 int pf_ctx_set_tile2(pf_ctx* ctx, int rows, int depth);
 int pf_default_rows2(int nx, int ny, int storage);
+int pf_ctx_set_tile3(pf_ctx* ctx, int rows, int depth);
+int pf_default_rows3(int nx, int ny, int storage);
 
 /* ------------------------------------------------------------------------
  * Batched form: many independent simulations of uneven size advanced by one
@@ -201,9 +218,10 @@ void pf_batch_destroy(pf_batch* batch);
 size_t pf_batch_free_bytes(pf_batch* batch);
 /* Rows per strip task and prefetch depth (2, 3 or 6) of the batched step. */
 int pf_batch_set_tile(pf_batch* batch, int rows, int depth);
-/* Steps per launch: 1, or 2 (the two-step march of pf_step2_*).  With 2 a
- * launch advances each active job by two steps, or by one where only one
- * remains or where the job's next step is a nucleation step. */
+/* Steps per launch: 1, 2 or 3 (the marches of pf_step2_* / pf_step3_*).  A
+ * launch advances each active job by up to that many steps: fewer where fewer
+ * remain or where a nucleation step falls before the last of them (the
+ * nucleation pass runs after the launch's last step). */
 int pf_batch_set_launch_steps(pf_batch* batch, int steps);
 /* Admit a job; fields are HOST arrays of the batch's storage type, (Nx+2)*(Ny+2) each.
  * Returns PF_ERR_OVERFLOW when the pool has no room (the caller may retry later). */

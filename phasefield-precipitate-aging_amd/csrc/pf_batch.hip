@@ -1,23 +1,23 @@
 // pf_batch.hip -- many independent simulations of uneven size advanced by one
-// persistent launch per step, or per two steps.
+// persistent launch per one, two or three steps.
 //
 // Layout: one device pool holds every job's two buffer sets (ping-pong), each
 // set four planes (x_Cr, x_Nb, phi_delta, phi_Laves) of pitch x (Ny + 2) cells
 // with the ghost ring, 256-byte aligned.  A device job table describes each job
 // (pool offsets, size, tiling, kinetic parameters, step counter, target) and a
-// wave table maps every strip task (job, 60-column strip, row chunk) of the
+// wave table maps every strip task (job, 58-column strip, row chunk) of the
 // current mix to its job.  The step kernel's grid is fixed (the GPU's resident
 // capacity), so a captured hipGraph of several launches stays valid when jobs
 // are admitted or retired: only the device tables change between replays.
 //
 // Per launch: step kernel (on every active job's strips, reading the job's
-// current set and writing the other: pf::march2 when the job advances two
-// steps this launch, pf::march for one), then the nucleation kernel (jobs
-// whose schedule is due on the last step of the launch; centre-only embeds),
-// whose last block advances every active job's step counter, retires jobs
-// that reached their target and decides each job's next advance.  A job
-// advances two steps per launch when two-step launches are on, two steps
-// remain and no nucleation step falls before the second of them.
+// current set and writing the other: pf::marchk when the job advances two or
+// three steps this launch, pf::march for one), then the nucleation kernel
+// (jobs whose schedule is due on the last step of the launch; centre-only
+// embeds), whose last block advances every active job's step counter, retires
+// jobs that reached their target and decides each job's next advance.  A job
+// advances k steps in a launch (k up to the batch's steps per launch) when k
+// steps remain and no nucleation step falls before the last of them.
 // A job's arithmetic and tiling depend only on the job, so its result is the
 // same bits alone or in any mix.
 #include <hip/hip_runtime.h>
@@ -42,8 +42,8 @@ struct BatchJob {
   int active;            // 1 while step < target
   int nuc_every;         // 0 = no nucleation
   int parity;            // buffer set holding the current fields
-  int pair;              // this launch advances two steps (pf::march2)
-  int fuse;              // two-step launches enabled for the batch
+  int adv;               // steps this launch advances the job (1, 2 or 3)
+  int max_adv;           // steps per launch of the batch (1 where the grid is too small)
   int pad;
   StepParams P;
   NucParams N;
@@ -89,22 +89,25 @@ __global__ __launch_bounds__(WAVE * WPB) void batch_step_kernel(const BatchJob* 
     const T* in = pool + J.off[par];
     T* out = pool + J.off[par ^ 1];
     const uint64_t pl = J.plane;
-    if (J.pair) {
-      if (J.bc == 0)
-        march2<T, 0, DEPTH>(in, in + pl, in + 2 * pl, in + 3 * pl, out, out + pl, out + 2 * pl,
-                            out + 3 * pl, J.Nx, J.Ny, J.pitch, strip, chunk, J.rows, J.P);
-      else
-        march2<T, 1, DEPTH>(in, in + pl, in + 2 * pl, in + 3 * pl, out, out + pl, out + 2 * pl,
-                            out + 3 * pl, J.Nx, J.Ny, J.pitch, strip, chunk, J.rows, J.P);
-    } else if (J.bc == 0) {
-      march<T, 0, false, false, DEPTH, STRIP2>(in, in + pl, in + 2 * pl, in + 3 * pl, out,
-                                               out + pl, out + 2 * pl, out + 3 * pl, J.Nx, J.Ny,
-                                               J.pitch, strip, chunk, J.rows, J.P, N, S);
+#define PF_BATCH_MARCH(BCV)                                                                  \
+  if (J.adv == 3)                                                                            \
+    marchk<T, BCV, 3, DEPTH, STRIP3>(in, in + pl, in + 2 * pl, in + 3 * pl, out, out + pl,     \
+                                     out + 2 * pl, out + 3 * pl, J.Nx, J.Ny, J.pitch, strip,   \
+                                     chunk, J.rows, J.P);                                      \
+  else if (J.adv == 2)                                                                       \
+    marchk<T, BCV, 2, DEPTH, STRIP3>(in, in + pl, in + 2 * pl, in + 3 * pl, out, out + pl,     \
+                                     out + 2 * pl, out + 3 * pl, J.Nx, J.Ny, J.pitch, strip,   \
+                                     chunk, J.rows, J.P);                                      \
+  else                                                                                       \
+    march<T, BCV, false, false, DEPTH, STRIP3>(in, in + pl, in + 2 * pl, in + 3 * pl, out,     \
+                                               out + pl, out + 2 * pl, out + 3 * pl, J.Nx,    \
+                                               J.Ny, J.pitch, strip, chunk, J.rows, J.P, N, S)
+    if (J.bc == 0) {
+      PF_BATCH_MARCH(0);
     } else {
-      march<T, 1, false, false, DEPTH, STRIP2>(in, in + pl, in + 2 * pl, in + 3 * pl, out,
-                                               out + pl, out + 2 * pl, out + 3 * pl, J.Nx, J.Ny,
-                                               J.pitch, strip, chunk, J.rows, J.P, N, S);
+      PF_BATCH_MARCH(1);
     }
+#undef PF_BATCH_MARCH
   }
 }
 
@@ -113,9 +116,12 @@ __global__ __launch_bounds__(WAVE * WPB) void batch_step_kernel(const BatchJob* 
 // then the last block advances the step counters and picks each job's next
 // advance (a two-step launch never has a nucleation step before its second
 // step).
-__host__ __device__ __forceinline__ int next_pair(const BatchJob& J) {
-  return J.fuse && J.step + 2 <= J.target &&
-         !(J.nuc_every > 0 && J.step % (uint64_t)J.nuc_every == 0);
+__host__ __device__ __forceinline__ int next_adv(const BatchJob& J) {
+  int k = 1;
+  while (k < J.max_adv && J.step + k + 1 <= J.target &&
+         !(J.nuc_every > 0 && (J.step + k - 1) % (uint64_t)J.nuc_every == 0))
+    k++;
+  return k;
 }
 
 template <typename T>
@@ -128,12 +134,12 @@ __global__ __launch_bounds__(WAVE * WPB) void batch_nuc_advance_kernel(
   for (; w < last; w += stride) {
     const int j = __builtin_amdgcn_readfirstlane(wave_job[w]);
     const BatchJob& J = jobs[j];
-    const uint64_t s_last = J.step + (J.pair ? 1 : 0);  // the step whose output this is
+    const uint64_t s_last = J.step + J.adv - 1;  // the step whose output this is
     if (!J.active || J.nuc_every <= 0 || (s_last % (uint64_t)J.nuc_every) != 0) continue;
     const int local = w - J.wave0;
     const int strip = local % J.strips, chunk = local / J.strips;
-    const int gx = strip * STRIP2 + lane - 1;
-    if (lane < 1 || lane > STRIP2 || gx >= J.Nx) continue;
+    const int gx = strip * STRIP3 + lane - 1;
+    if (lane < 1 || lane > STRIP3 || gx >= J.Nx) continue;
     T* out = pool + J.off[J.parity ^ 1];
     const uint64_t pl = J.plane;
     NucParams N = J.N;
@@ -172,10 +178,10 @@ __global__ __launch_bounds__(WAVE * WPB) void batch_nuc_advance_kernel(
     for (int j = threadIdx.x; j < hdr->njobs; j += blockDim.x) {
       BatchJob& J = jobs[j];
       if (J.active) {
-        J.step += J.pair ? 2 : 1;
+        J.step += J.adv;
         J.parity ^= 1;
         if (J.step >= J.target) J.active = 0;
-        J.pair = next_pair(J);
+        J.adv = next_adv(J);
       }
     }
     __syncthreads();
@@ -267,7 +273,7 @@ struct pf_batch {
   hipStream_t stream = nullptr;
   int grid_blocks = 0;
   int rows = 16, depth = 3;
-  int fuse = 0;  // two-step launches
+  int launch_steps = 1;  // steps per launch (1, 2 or 3)
   // host mirror
   std::vector<pf::BatchJob> jobs;       // slot -> job (slots are reused)
   std::vector<int> slot_live;           // 1 if the slot holds a job
@@ -343,7 +349,7 @@ int sync_jobs_from_device(pf_batch* b) {
       b->jobs[j].step = dev[j].step;
       b->jobs[j].active = dev[j].active;
       b->jobs[j].parity = dev[j].parity;
-      b->jobs[j].pair = dev[j].pair;
+      b->jobs[j].adv = dev[j].adv;
     }
   return PF_OK;
 }
@@ -401,11 +407,11 @@ int pf_batch_create(pf_batch** out, int storage, size_t pool_bytes, int max_jobs
   b->elem = storage == PF_STORAGE_F32 ? 4 : 8;
   b->pool_elems = pool_bytes / b->elem;
   b->max_jobs = max_jobs;
-  // every wave covers >= 60 x 4 cells of one job; jobs add at most one
+  // every wave covers >= 58 x 4 cells of one job; jobs add at most one
   // partial strip and chunk per dimension
   const size_t cells = b->pool_elems / 8;
   b->max_waves = (int)std::min<size_t>((size_t)1 << 28,
-                                       cells / (pf::STRIP2 * 4) + (size_t)max_jobs * 64);
+                                       cells / (pf::STRIP3 * 4) + (size_t)max_jobs * 64);
   hipError_t e;
   if ((e = hipMalloc(&b->pool, b->pool_elems * b->elem)) != hipSuccess ||
       (e = hipMalloc((void**)&b->d_jobs, (size_t)max_jobs * sizeof(pf::BatchJob))) != hipSuccess ||
@@ -466,9 +472,9 @@ int pf_batch_set_tile(pf_batch* b, int rows, int depth) {
 
 int pf_batch_set_launch_steps(pf_batch* b, int steps) {
   if (!b) return bfail(PF_ERR_ARG, "null batch");
-  if (steps != 1 && steps != 2) return bfail(PF_ERR_ARG, "steps per launch must be 1 or 2");
+  if (steps < 1 || steps > 3) return bfail(PF_ERR_ARG, "steps per launch must be 1, 2 or 3");
   if (int r = sync_jobs_from_device(b)) return r;
-  b->fuse = steps == 2 ? 1 : 0;
+  b->launch_steps = steps;
   b->dirty = true;
   return PF_OK;
 }
@@ -575,10 +581,10 @@ int pf_batch_commit(pf_batch* b) {
       continue;
     }
     J.rows = b->rows;
-    J.strips = (J.Nx + pf::STRIP2 - 1) / pf::STRIP2;
+    J.strips = (J.Nx + pf::STRIP3 - 1) / pf::STRIP3;
     J.chunks = (J.Ny + J.rows - 1) / J.rows;
-    J.fuse = b->fuse && J.Nx >= 3 && J.Ny >= 3;
-    J.pair = J.active ? pf::next_pair(J) : 0;
+    J.max_adv = (J.Nx >= 3 && J.Ny >= 3) ? b->launch_steps : 1;
+    J.adv = J.active ? pf::next_adv(J) : 1;
     J.wave0 = (int)wave_job.size();
     if (!J.active) {
       J.wave0 = 0;
@@ -737,7 +743,7 @@ int pf_batch_waves(pf_batch* b) {
   int total = 0;
   for (size_t j = 0; j < b->jobs.size(); j++)
     if (b->slot_live[j] && b->jobs[j].active)
-      total += ((b->jobs[j].Nx + pf::STRIP2 - 1) / pf::STRIP2) *
+      total += ((b->jobs[j].Nx + pf::STRIP3 - 1) / pf::STRIP3) *
                ((b->jobs[j].Ny + b->rows - 1) / b->rows);
   return total;
 }

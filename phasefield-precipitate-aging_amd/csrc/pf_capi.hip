@@ -80,12 +80,13 @@ int step_t(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const T* i
 }
 
 template <typename T>
-int step2_t(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const T* iCr,
+int stepk_t(int k, pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const T* iCr,
             const T* iNb, const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd, T* opl,
             void* stream) {
   if (!ctx || !prm) return fail(PF_ERR_ARG, "null context or parameters");
   if (int r = check_grid(nx, ny)) return r;
-  if (nx < 5 || ny < 5) return fail(PF_ERR_ARG, "two-step kernel needs at least 3x3 interior cells");
+  if (nx < 5 || ny < 5)
+    return fail(PF_ERR_ARG, "multi-step kernels need at least 3x3 interior cells");
   if (int r = check_ptrs(iCr, iNb, ipd, ipl)) return r;
   if (int r = check_ptrs(oCr, oNb, opd, opl)) return r;
   if (bc != PF_BC_NOFLUX && bc != PF_BC_PERIODIC) return fail(PF_ERR_ARG, "unknown boundary mode");
@@ -94,12 +95,22 @@ int step2_t(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const T* 
     return fail(PF_ERR_ARG, "step cannot run in place: old and new buffers must differ");
   const double* mk = prm->mask;
   if (mk[0] != 0.0 || mk[2] != 0.0 || mk[6] != 0.0 || mk[8] != 0.0)
-    return fail(PF_ERR_ARG, "two-step kernel takes five-point masks only");
+    return fail(PF_ERR_ARG, "multi-step kernels take five-point masks only");
   const pf::StepParams P = pf::make_step(prm->dt, prm->DCr, prm->DNb, prm->kappa, prm->omega,
                                          prm->Lmob, prm->alpha, prm->mask);
-  hipError_t e = pf::launch_step2<T>(nx, ny, bc, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl,
+  hipError_t e = pf::launch_stepk<T>(k, nx, ny, bc, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl,
                                      ctx->tile, (hipStream_t)stream);
-  if (e != hipSuccess) return hip_fail(e, "two-step kernel");
+  if (e != hipSuccess) return hip_fail(e, k == 3 ? "three-step kernel" : "two-step kernel");
+  return PF_OK;
+}
+
+int set_tile_k(pf_ctx* ctx, int k, int rows, int depth) {
+  if (!ctx) return fail(PF_ERR_ARG, "null context");
+  if (rows < 0 || rows > 1 << 20) return fail(PF_ERR_ARG, "rows out of range");
+  if (depth != 0 && depth != 2 && depth != 3 && depth != 6)
+    return fail(PF_ERR_ARG, "prefetch depth must be 2, 3 or 6 (0 = default)");
+  (k == 3 ? ctx->tile.rows3 : ctx->tile.rows2) = rows;
+  (k == 3 ? ctx->tile.depth3 : ctx->tile.depth2) = depth ? depth : 3;
   return PF_OK;
 }
 
@@ -178,32 +189,6 @@ int pf_ctx_set_tile(pf_ctx* ctx, int rows, int xcd_remap, int depth) {
 
 int pf_default_rows(int nx, int ny) { return pf::default_rows(nx, ny); }
 
-int pf_ctx_set_tile2(pf_ctx* ctx, int rows, int depth) {
-  if (!ctx) return fail(PF_ERR_ARG, "null context");
-  if (rows < 0 || rows > 1 << 20) return fail(PF_ERR_ARG, "rows out of range");
-  if (depth != 0 && depth != 2 && depth != 3 && depth != 6)
-    return fail(PF_ERR_ARG, "prefetch depth must be 2, 3 or 6 (0 = default)");
-  ctx->tile.rows2 = rows;
-  ctx->tile.depth2 = depth ? depth : 3;
-  return PF_OK;
-}
-
-int pf_default_rows2(int nx, int ny, int storage) {
-  return pf::default_rows2(nx, ny, storage == PF_STORAGE_F32 ? 4 : 8);
-}
-
-int pf_step2_f64(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const double* a,
-                 const double* b, const double* c, const double* d, double* oa, double* ob,
-                 double* oc, double* od, void* stream) {
-  return step2_t<double>(ctx, nx, ny, bc, prm, a, b, c, d, oa, ob, oc, od, stream);
-}
-
-int pf_step2_f32(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const float* a,
-                 const float* b, const float* c, const float* d, float* oa, float* ob,
-                 float* oc, float* od, void* stream) {
-  return step2_t<float>(ctx, nx, ny, bc, prm, a, b, c, d, oa, ob, oc, od, stream);
-}
-
 int pf_step_f64(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const double* a,
                 const double* b, const double* c, const double* d, double* oa, double* ob,
                 double* oc, double* od, const pf_nuc_params* nuc, void* stream) {
@@ -214,6 +199,40 @@ int pf_step_f32(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const
                 const float* b, const float* c, const float* d, float* oa, float* ob, float* oc,
                 float* od, const pf_nuc_params* nuc, void* stream) {
   return step_t<float>(ctx, nx, ny, bc, prm, a, b, c, d, oa, ob, oc, od, nuc, stream);
+}
+
+int pf_ctx_set_tile2(pf_ctx* ctx, int rows, int depth) { return set_tile_k(ctx, 2, rows, depth); }
+int pf_ctx_set_tile3(pf_ctx* ctx, int rows, int depth) { return set_tile_k(ctx, 3, rows, depth); }
+
+int pf_default_rows2(int nx, int ny, int storage) {
+  return pf::default_rows_k(2, nx, ny, storage == PF_STORAGE_F32 ? 4 : 8);
+}
+int pf_default_rows3(int nx, int ny, int storage) {
+  return pf::default_rows_k(3, nx, ny, storage == PF_STORAGE_F32 ? 4 : 8);
+}
+
+int pf_step2_f64(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const double* a,
+                 const double* b, const double* c, const double* d, double* oa, double* ob,
+                 double* oc, double* od, void* stream) {
+  return stepk_t<double>(2, ctx, nx, ny, bc, prm, a, b, c, d, oa, ob, oc, od, stream);
+}
+
+int pf_step2_f32(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const float* a,
+                 const float* b, const float* c, const float* d, float* oa, float* ob,
+                 float* oc, float* od, void* stream) {
+  return stepk_t<float>(2, ctx, nx, ny, bc, prm, a, b, c, d, oa, ob, oc, od, stream);
+}
+
+int pf_step3_f64(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const double* a,
+                 const double* b, const double* c, const double* d, double* oa, double* ob,
+                 double* oc, double* od, void* stream) {
+  return stepk_t<double>(3, ctx, nx, ny, bc, prm, a, b, c, d, oa, ob, oc, od, stream);
+}
+
+int pf_step3_f32(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const float* a,
+                 const float* b, const float* c, const float* d, float* oa, float* ob,
+                 float* oc, float* od, void* stream) {
+  return stepk_t<float>(3, ctx, nx, ny, bc, prm, a, b, c, d, oa, ob, oc, od, stream);
 }
 
 int pf_step_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream) {
@@ -229,17 +248,25 @@ int pf_step_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream
   return PF_OK;
 }
 
-int pf_step2_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream) {
+static int stepk_loop(int k, pf_ctx* ctx, int n, const pf_step_item* items, void* stream) {
   if (n < 0 || (n > 0 && !items)) return fail(PF_ERR_ARG, "bad item list");
   for (int i = 0; i < n; i++) {
     const pf_step_item& it = items[i];
-    if (int r = step2_t<double>(ctx, it.nx, it.ny, it.bc, it.prm, it.old_fields[0],
+    if (int r = stepk_t<double>(k, ctx, it.nx, it.ny, it.bc, it.prm, it.old_fields[0],
                                 it.old_fields[1], it.old_fields[2], it.old_fields[3],
                                 it.new_fields[0], it.new_fields[1], it.new_fields[2],
                                 it.new_fields[3], stream))
       return r;
   }
   return PF_OK;
+}
+
+int pf_step2_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream) {
+  return stepk_loop(2, ctx, n, items, stream);
+}
+
+int pf_step3_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream) {
+  return stepk_loop(3, ctx, n, items, stream);
 }
 
 int pf_nucleation_f64(pf_ctx* ctx, int nx, int ny, double* a, double* b, double* c, double* d,

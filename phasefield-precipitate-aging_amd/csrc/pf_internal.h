@@ -62,6 +62,8 @@ struct TileConfig {
   int depth = 3;  // rows of loads in flight ahead of the computed row (2, 3 or 6)
   int rows2 = 0;  // the same for the two-step kernel
   int depth2 = 3;
+  int rows3 = 0;  // and for the three-step kernel
+  int depth3 = 3;
 };
 
 hipError_t scratch_alloc(Scratch& s);
@@ -73,11 +75,11 @@ hipError_t launch_step(int nx, int ny, int bc, const StepParams& P, const T* iCr
                        const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd, T* opl,
                        const NucParams* nuc, const Scratch& S, const TileConfig& tc,
                        hipStream_t st);
-// two steps per launch (five-point mask, no nucleation); grids need Nx, Ny >= 3
+// k = 2 or 3 steps per launch (five-point mask, no nucleation); grids need Nx, Ny >= 3
 template <typename T>
-hipError_t launch_step2(int nx, int ny, int bc, const StepParams& P, const T* iCr, const T* iNb,
-                        const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd, T* opl,
-                        const TileConfig& tc, hipStream_t st);
+hipError_t launch_stepk(int k, int nx, int ny, int bc, const StepParams& P, const T* iCr,
+                        const T* iNb, const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd,
+                        T* opl, const TileConfig& tc, hipStream_t st);
 template <typename T>
 hipError_t launch_nucleation(int nx, int ny, T* xCr, T* xNb, T* pd, T* pl, const NucParams& N,
                              const Scratch& S, hipStream_t st);
@@ -93,7 +95,7 @@ hipError_t launch_thermo_eval(int what, int n, const double* in, double* out, hi
 hipError_t launch_philox_eval(int n, const uint32_t* in, uint32_t* out, hipStream_t st);
 
 int default_rows(int nx, int ny);
-int default_rows2(int nx, int ny, int storage_bytes);
+int default_rows_k(int k, int nx, int ny, int storage_bytes);
 
 NucParams make_nuc(double sigma_del, double sigma_lav, double unit_a, double dx, double dy,
                    double dz, double dt, uint64_t seed, uint64_t call, uint32_t instance,

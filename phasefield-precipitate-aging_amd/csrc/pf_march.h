@@ -30,6 +30,7 @@ namespace pf {
 constexpr int WAVE = 64;
 constexpr int STRIP = WAVE - 2;   // output columns per strip
 constexpr int STRIP2 = WAVE - 4;  // output columns per strip of the two-step march
+constexpr int STRIP3 = WAVE - 6;  // and of the three-step march (the batched form's strips)
 constexpr int WPB = 4;           // waves (strips) per block: adjacent strips of one row chunk
 constexpr int UNROLL = 6;        // multiple of the 3-row window and of every ring depth
 
@@ -408,48 +409,59 @@ __device__ __forceinline__ void march(
 
 
 // ---------------------------------------------------------------------------
-// Two explicit steps per pass (temporal blocking).  The reference advances
-// one step per sequence of launches and so reads and writes every field once
-// per step; here a wave marches its strip through two time levels at once, so
-// the old fields are read once and the fields two steps later written once:
-// half the HBM traffic per step of the one-step march.
+// K explicit steps per pass (temporal blocking, K = 2 or 3).  The reference
+// advances one step per sequence of launches and so reads and writes every
+// field once per step; here a wave marches its strip through K time levels at
+// once, so the old fields are read once and the fields K steps later written
+// once: 1/K of the HBM traffic per step of the one-step march.
 //
-// A strip has 60 output columns and a two-column halo each side (lanes 0-1
-// and 62-63).  Stage A (the first step) is computed on every lane and is valid
-// on lanes 1..62; stage B (the second step) reads stage A's values through the
-// same DPP shifts and is valid, and stored, on lanes 2..61.  In y, stage A runs
-// one row ahead of stage B, from row y0-1 to row y1, so the chunk reads old
-// rows y0-2 .. y1+1.
+// A strip has 64 - 2K output columns and a K-column halo each side.  Level
+// l (1..K) is computed on every lane and is valid on lanes l .. 63-l; level K
+// is stored, on lanes K .. 63-K.  Level l reads level l-1 through the same DPP
+// shifts as the one-step march.
 //
-// The intermediate level is exactly what a first one-step launch would have
-// stored: stage-A values are rounded to the storage type before stage B uses
-// them, and stage-A values at positions outside the interior are the images
-// the one-step kernel writes into the ghost ring.  Periodic images come out
-// of the index map directly (a halo lane at gx = -1 reads the neighbourhood of
-// column Nx-1).  No-flux images are copies of the adjacent interior cell's
-// stage-A value: taken from the neighbouring lane in x, and from the adjacent
-// stage-A row in y.  So two steps here give the same result as two one-step
-// launches.  Five-point masks, no nucleation (the caller runs nucleation
-// steps one at a time).
-
-template <typename T, int BC, int DEPTH>
-__device__ __forceinline__ void march2(
+// The levels form a pipeline along the march: at iteration t level 0 (the old
+// fields, loaded) reaches row y0-K+t and level l computes row y0-K+t-l from
+// level l-1's rows y0-K+t-l-1 .. y0-K+t-l+1, the last of which level l-1 has
+// just produced.  Level l starts at iteration 2l, so the chunk reads old rows
+// y0-K .. y1-1+K and level l covers rows y0-(K-l) .. y1-1+(K-l).  Each level
+// keeps a three-row window in registers, row computed at iteration t in slot
+// t % 3; the first 2K iterations (the prologue) are unrolled at compile time.
+//
+// The intermediate levels are exactly what one-step launches would have
+// stored: values are rounded to the storage type before the next level uses
+// them, and values at positions outside the interior are the images the
+// one-step kernel writes into the ghost ring.  Periodic images come out of the
+// index map directly (a halo lane at gx = -1 reads the neighbourhood of column
+// Nx-1).  No-flux images are copies of the adjacent interior cell's value at
+// the same level: taken from the neighbouring lane in x, and from the adjacent
+// row of the level's window in y.  So K steps here give the same result as K
+// one-step launches.  Five-point masks, no nucleation (the caller runs
+// nucleation steps one at a time).
+template <typename T, int BC, int K, int DEPTH, int SW = WAVE - 2 * K>
+__device__ __forceinline__ void marchk(
     const T* __restrict__ iCr, const T* __restrict__ iNb, const T* __restrict__ ipd,
     const T* __restrict__ ipl, T* __restrict__ oCr, T* __restrict__ oNb, T* __restrict__ opd,
     T* __restrict__ opl, int Nx, int Ny, int pitch, int strip, int chunk, int rows,
     const StepParams& P) {
+  static_assert(K >= 2 && K <= 3, "two or three steps per pass");
   static_assert(UNROLL % DEPTH == 0, "ring depth must divide the unroll factor");
+  // SW output columns per strip (64 - 2K, or fewer where the strips are laid
+  // out for a deeper march); lanes K .. K+SW-1 store
+  static_assert(SW >= 1 && SW <= WAVE - 2 * K, "strip width");
   const int lane = threadIdx.x & (WAVE - 1);
 
-  const int gx = strip * STRIP2 + lane - 2;  // interior column of this lane (may be a halo)
+  const int gx = strip * SW + lane - K;  // interior column of this lane (may be a halo)
   const int px = 1 + bc_map<BC>(gx, Nx);
-  const bool out_lane = (lane >= 2) && (lane <= STRIP2 + 1) && (gx < Nx);
+  const bool out_lane = (lane >= K) && (lane < K + SW) && (gx < Nx);
   // does this wave hold a no-flux image column (gx = -1 or gx = Nx)?
-  const bool x_images = BC == 0 && (strip == 0 || strip * STRIP2 + WAVE - 3 >= Nx);
+  const bool x_images = BC == 0 && (strip == 0 || strip * SW + WAVE - 1 - K >= Nx);
 
   const int y0 = chunk * rows;
   const int y1 = min(y0 + rows, Ny);
   if (y0 >= Ny) return;
+  const int nrows = y1 - y0;
+  const int y_last = y1 - 1 + K;  // last old row the chunk reads
 
   const uint32_t in_lane = (uint32_t)px * (uint32_t)sizeof(T);
   const uint32_t out_lane_b = (uint32_t)(1 + gx) * (uint32_t)sizeof(T);
@@ -459,10 +471,16 @@ __device__ __forceinline__ void march2(
   };
   const double* m = P.m;
 
-  // Stage A at the centre of three derived old rows: the first step's values,
-  // rounded to storage, with no-flux x images, derived for stage B.
-  auto stage_a = [&](const Point& Sth, const Point& C, const Centre& cc, const Point& Nth,
-                     Point& v, Centre& c) {
+  // windows: level l (0 = old fields) row of iteration t in slot t % 3
+  Point win[K][3];
+  Centre cen[K][3];
+  Raw<T> ring[DEPTH];
+
+  // One step at the centre of level `lv`'s window (slots behind / centre /
+  // ahead): the next level's values, rounded to storage, with no-flux x
+  // images, derived into `v`, `c`.
+  auto level_up = [&](const Point& Sth, const Point& C, const Centre& cc, const Point& Nth,
+                      Point& v, Centre& c) {
     const Point W = shift_left(C), E = shift_right(C);
     double lap[4];
     laplacians<false>(m, Sth, W, C, E, Nth, Sth, Sth, Nth, Nth, lap);
@@ -488,53 +506,69 @@ __device__ __forceinline__ void march2(
     derive<T>(r, v, c);
   };
 
-  // old rows: row r in slot (r - y0 + 2) % 3; stage-A rows: row r in slot (r - y0 + 1) % 3
-  Point a[3], b[3];
-  Centre ac[3], bcn[3];
-  Raw<T> ring[DEPTH];  // old rows y0+2+j
-  {
-    Raw<T> r;
-    load(r, y0 - 2);
-    derive<T>(r, a[0], ac[0]);
-    load(r, y0 - 1);
-    derive<T>(r, a[1], ac[1]);
-    load(r, y0);
-    derive<T>(r, a[2], ac[2]);
+  // Levels 1..K-1 at iteration t (slots are compile-time in both callers).
+  // Level l computes row y0-K+t-l; a no-flux image row (-1 or Ny) becomes a
+  // copy of the adjacent interior row of the same level: row -1 when row 0
+  // arrives one iteration later (always within the prologue, `top`), row Ny
+  // from row Ny-1 at once.
+  auto middle_levels = [&](int t, int sa, int sc, int sb, int lmax_active, bool top) {
 #pragma unroll
-    for (int j = 0; j < DEPTH; j++) load(ring[j], min(y0 + 2 + j, y1 + 1));
-    stage_a(a[0], a[1], ac[1], a[2], b[0], bcn[0]);  // stage-A row y0-1
-    load(r, y0 + 1);
-    derive<T>(r, a[0], ac[0]);
-    stage_a(a[1], a[2], ac[2], a[0], b[1], bcn[1]);  // stage-A row y0
-    if (BC == 0 && y0 == 0) {  // no-flux image row -1 = row 0
-      b[0] = b[1];
-      bcn[0] = bcn[1];
+    for (int l = 1; l < K; l++) {
+      if (l > lmax_active) break;
+      level_up(win[l - 1][sb], win[l - 1][sc], cen[l - 1][sc], win[l - 1][sa], win[l][sa],
+               cen[l][sa]);
+      if (BC == 0) {
+        const int row = y0 - K + t - l;
+        if (top && row == 0) {  // the image row -1 (when this chunk has one) is in slot sc
+          win[l][sc] = win[l][sa];
+          cen[l][sc] = cen[l][sa];
+        }
+        if (row == Ny) {
+          win[l][sa] = win[l][sc];
+          cen[l][sa] = cen[l][sc];
+        }
+      }
+    }
+  };
+
+  // ---- prologue: iterations t = 0 .. 2K-1 (level K not yet active)
+  {
+#pragma unroll
+    for (int t = 0; t < 2 * K; t++) {
+      const int sa = t % 3, sc = (t + 2) % 3, sb = (t + 1) % 3;
+      Raw<T> r;
+      load(r, y0 - K + t);
+      derive<T>(r, win[0][sa], cen[0][sa]);
+      if (t == 2) {  // start the ring once the first window is full
+#pragma unroll
+        for (int j = 0; j < DEPTH; j++) load(ring[j], min(y0 + K + j, y_last));
+      }
+      // level l is active from iteration 2l on
+      middle_levels(t, sa, sc, sb, t / 2, true);
     }
   }
 
-  for (int yb = y0;; yb += UNROLL) {
+  // ---- main loop: iteration t = 2K + i, output row y = y0 + i
+  for (int ib = 0;; ib += UNROLL) {
 #pragma unroll
     for (int k = 0; k < UNROLL; k++) {
-      const int y = yb + k;  // output row of stage B
-      if (y >= y1) return;
-      const int a0 = (k + 2) % 3, a1 = k % 3, a2 = (k + 1) % 3;  // old rows y, y+1, y+2
-      const int b0 = k % 3, b1 = (k + 1) % 3, b2 = (k + 2) % 3;  // stage-A rows y-1, y, y+1
+      if (ib + k >= nrows) return;
+      const int y = y0 + ib + k;
+      const int t = 2 * K + ib + k;
+      const int sa = (2 * K + k) % 3, sc = (2 * K + k + 2) % 3, sb = (2 * K + k + 1) % 3;
       const int rs = k % DEPTH;
-      derive<T>(ring[rs], a[a2], ac[a2]);
-      load(ring[rs], min(y + 2 + DEPTH, y1 + 1));
-
-      stage_a(a[a0], a[a1], ac[a1], a[a2], b[b2], bcn[b2]);  // stage-A row y+1
-      if (BC == 0 && y + 1 == Ny) {  // no-flux image row Ny = row Ny-1
-        b[b2] = b[b1];
-        bcn[b2] = bcn[b1];
-      }
-
-      const Point W = shift_left(b[b1]), E = shift_right(b[b1]);
+      derive<T>(ring[rs], win[0][sa], cen[0][sa]);  // old row y + K
+      load(ring[rs], min(y + K + DEPTH, y_last));
+      middle_levels(t, sa, sc, sb, K - 1, false);
+      // level K: row y from level K-1's rows y-1 (behind), y, y+1 (ahead)
+      const Point& Sth = win[K - 1][sb];
+      const Point& C = win[K - 1][sc];
+      const Point& Nth = win[K - 1][sa];
+      const Point W = shift_left(C), E = shift_right(C);
       double lap[4];
-      laplacians<false>(m, b[b0], W, b[b1], E, b[b2], b[b0], b[b0], b[b2], b[b2], lap);
+      laplacians<false>(m, Sth, W, C, E, Nth, Sth, Sth, Nth, Nth, lap);
       double nCr, nNb, npd, npl;
-      update_cell(P, lap, b[b1], bcn[b1], nCr, nNb, npd, npl);
-
+      update_cell(P, lap, C, cen[K - 1][sc], nCr, nNb, npd, npl);
       if (out_lane) {
         store_row(oCr, oNb, opd, opl, (size_t)(1 + y) * row_b, out_lane_b, nCr, nNb, npd, npl);
         store_ghost_images<T, BC>(oCr, oNb, opd, opl, gx, y, Nx, Ny, pitch, nCr, nNb, npd, npl);

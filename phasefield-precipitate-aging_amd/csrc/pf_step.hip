@@ -101,10 +101,10 @@ hipError_t launch_step(int nx, int ny, int bc, const StepParams& P, const T* iCr
 #undef PF_LAUNCH
 }
 
-// ---- two steps per launch (march2) ----
+// ---- K steps per launch (marchk, K = 2 or 3) ----
 
-template <typename T, int BC, int DEPTH>
-__global__ __launch_bounds__(WAVE * WPB) void step2_kernel(
+template <typename T, int BC, int K, int DEPTH>
+__global__ __launch_bounds__(WAVE * WPB) void stepk_kernel(
     const T* __restrict__ iCr, const T* __restrict__ iNb, const T* __restrict__ ipd,
     const T* __restrict__ ipl, T* __restrict__ oCr, T* __restrict__ oNb, T* __restrict__ opd,
     T* __restrict__ opl, int Nx, int Ny, int pitch, int strips, int chunks, int rows, int remap,
@@ -114,60 +114,72 @@ __global__ __launch_bounds__(WAVE * WPB) void step2_kernel(
   const int b = (remap ? xcd_remap(blockIdx.x, nblocks) : (int)blockIdx.x) * WPB +
                 __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));  // wave-uniform
   if (b >= nwaves) return;
-  march2<T, BC, DEPTH>(iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny, pitch, b % strips,
-                       b / strips, rows, P);
+  marchk<T, BC, K, DEPTH>(iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny, pitch, b % strips,
+                          b / strips, rows, P);
 }
 
-int default_rows2(int nx, int ny, int storage_bytes) {
-  // Stage A recomputes two rows per chunk, so chunks are longer than the
-  // one-step kernel's while the grid still fills the chip.  FP64 storage is
-  // near the HBM bound and prefers more waves in flight (16 rows at 4096^2);
-  // FP32 storage is bound by arithmetic and prefers less recomputation (32).
+int default_rows_k(int k, int nx, int ny, int storage_bytes) {
+  // The intermediate levels recompute (K-1)K rows per chunk in all, so chunks
+  // are longer than the one-step kernel's while the grid still fills the chip.
+  // Two steps: FP64 storage is near the HBM bound and prefers more waves in
+  // flight (16 rows at 4096^2); FP32 storage is bound by arithmetic and
+  // prefers less recomputation (32).  Three steps are bound by arithmetic in
+  // both and take 48 rows at 4096^2.
   const int Nx = nx - 2, Ny = ny - 2;
-  const int strips = (Nx + STRIP2 - 1) / STRIP2;
-  int rows = storage_bytes == 8 ? 16 : 32;
-  while (rows > 6 && (long)strips * ((Ny + rows - 1) / rows) < 8192) rows >>= 1;
+  const int sw = WAVE - 2 * k;
+  const int strips = (Nx + sw - 1) / sw;
+  int rows = k == 3 ? 48 : (storage_bytes == 8 ? 16 : 32);
+  const long min_waves = k == 3 ? 4096 : 8192;
+  while (rows > 6 && (long)strips * ((Ny + rows - 1) / rows) < min_waves) rows >>= 1;
   return rows;
 }
 
-template <typename T, int BC, int DEPTH>
-static hipError_t launch_step2_t(int nx, int ny, const StepParams& P, const T* iCr,
+template <typename T, int BC, int K, int DEPTH>
+static hipError_t launch_stepk_t(int nx, int ny, const StepParams& P, const T* iCr,
                                  const T* iNb, const T* ipd, const T* ipl, T* oCr, T* oNb,
-                                 T* opd, T* opl, const TileConfig& tc, hipStream_t st) {
+                                 T* opd, T* opl, int rows_cfg, int remap, hipStream_t st) {
   const int Nx = nx - 2, Ny = ny - 2;
-  const int rows = tc.rows2 > 0 ? tc.rows2 : default_rows2(nx, ny, (int)sizeof(T));
-  const int strips = (Nx + STRIP2 - 1) / STRIP2;
+  const int rows = rows_cfg > 0 ? rows_cfg : default_rows_k(K, nx, ny, (int)sizeof(T));
+  const int sw = WAVE - 2 * K;
+  const int strips = (Nx + sw - 1) / sw;
   const int chunks = (Ny + rows - 1) / rows;
   const int blocks = (strips * chunks + WPB - 1) / WPB;
-  hipLaunchKernelGGL((step2_kernel<T, BC, DEPTH>), dim3(blocks), dim3(WAVE * WPB), 0, st, iCr,
-                     iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny, nx, strips, chunks, rows,
-                     tc.remap, P);
+  hipLaunchKernelGGL((stepk_kernel<T, BC, K, DEPTH>), dim3(blocks), dim3(WAVE * WPB), 0, st,
+                     iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny, nx, strips, chunks, rows,
+                     remap, P);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_step2(int nx, int ny, int bc, const StepParams& P, const T* iCr, const T* iNb,
-                        const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd, T* opl,
-                        const TileConfig& tc, hipStream_t st) {
-#define PF_LAUNCH2(BCV, D)                                                                  \
-  return launch_step2_t<T, BCV, D>(nx, ny, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, tc, st)
-  if (bc == 0) {
-    if (tc.depth2 == 2) PF_LAUNCH2(0, 2);
-    if (tc.depth2 == 6) PF_LAUNCH2(0, 6);
-    PF_LAUNCH2(0, 3);
-  } else {
-    if (tc.depth2 == 2) PF_LAUNCH2(1, 2);
-    if (tc.depth2 == 6) PF_LAUNCH2(1, 6);
-    PF_LAUNCH2(1, 3);
+hipError_t launch_stepk(int k, int nx, int ny, int bc, const StepParams& P, const T* iCr,
+                        const T* iNb, const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd,
+                        T* opl, const TileConfig& tc, hipStream_t st) {
+  const int rows = k == 3 ? tc.rows3 : tc.rows2;
+  const int depth = k == 3 ? tc.depth3 : tc.depth2;
+#define PF_LAUNCHK(BCV, KV, D)                                                             \
+  return launch_stepk_t<T, BCV, KV, D>(nx, ny, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, \
+                                       rows, tc.remap, st)
+#define PF_LAUNCHK_D(BCV, KV)          \
+  do {                                 \
+    if (depth == 2) PF_LAUNCHK(BCV, KV, 2); \
+    if (depth == 6) PF_LAUNCHK(BCV, KV, 6); \
+    PF_LAUNCHK(BCV, KV, 3);            \
+  } while (0)
+  if (k == 3) {
+    if (bc == 0) PF_LAUNCHK_D(0, 3);
+    PF_LAUNCHK_D(1, 3);
   }
-#undef PF_LAUNCH2
+  if (bc == 0) PF_LAUNCHK_D(0, 2);
+  PF_LAUNCHK_D(1, 2);
+#undef PF_LAUNCHK_D
+#undef PF_LAUNCHK
 }
 
-template hipError_t launch_step2<double>(int, int, int, const StepParams&, const double*,
+template hipError_t launch_stepk<double>(int, int, int, int, const StepParams&, const double*,
                                          const double*, const double*, const double*, double*,
                                          double*, double*, double*, const TileConfig&,
                                          hipStream_t);
-template hipError_t launch_step2<float>(int, int, int, const StepParams&, const float*,
+template hipError_t launch_stepk<float>(int, int, int, int, const StepParams&, const float*,
                                         const float*, const float*, const float*, float*, float*,
                                         float*, float*, const TileConfig&, hipStream_t);
 

@@ -102,7 +102,7 @@ class Batch:
         check(blib().pf_batch_set_tile(self.h, rows, depth), "pf_batch_set_tile")
 
     def set_launch_steps(self, steps):
-        """1: one step per launch; 2: two (the temporally blocked march)."""
+        """Steps per launch: 1, 2 or 3 (2 and 3: the temporally blocked marches)."""
         check(blib().pf_batch_set_launch_steps(self.h, steps), "pf_batch_set_launch_steps")
         self.launch_steps = steps
 
@@ -130,8 +130,8 @@ class Batch:
 
     def run(self, launches, launches_per_graph=0):
         """Enqueue `launches` step launches (each advances every active job by
-        one step, or two with set_launch_steps(2)), replayed from a captured
-        graph of `launches_per_graph` launches when that is > 1."""
+        up to set_launch_steps() steps), replayed from a captured graph of
+        `launches_per_graph` launches when that is > 1."""
         check(blib().pf_batch_run(self.h, int(launches), int(launches_per_graph)),
               "pf_batch_run")
 

@@ -70,7 +70,7 @@ def request_desc(r, dx=0.25e-9):
     return s, make_desc(s, r.steps, nucleation=True, seed=r.seed, instance=r.rid)
 
 
-def serve(requests, pool_bytes, launches_per_graph=25, rows=16, depth=3, launch_steps=2,
+def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_steps=3,
           log=None, storage=PF_STORAGE_F32, time_limit=None):
     """Run the request stream; returns (requests, stats).  Each device round is
     one replay of a graph of `launches_per_graph` step launches, each advancing

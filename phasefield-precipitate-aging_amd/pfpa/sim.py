@@ -1,6 +1,6 @@
-"""Native stepping loop over the C-ABI.  Steps run two per launch (the
-temporally blocked kernel) wherever neither step of the pair is a nucleation
-step; nucleation steps, the ones the reference's schedule selects
+"""Native stepping loop over the C-ABI.  Steps run up to three per launch
+(the temporally blocked kernels) wherever no step of the group is a
+nucleation step; nucleation steps, the ones the reference's schedule selects
 (kernel_time % nuc_interval == 0, src/main.cpp:307-316), run one per launch
 with the nucleation pass fused."""
 import ctypes
@@ -24,7 +24,7 @@ class Simulation:
 
     def __init__(self, setup, fields, dtype=torch.float64, nucleation=False, seed=42,
                  nuc_every=None, instance=0, stream=None, ctx=None, nuc_dx=None,
-                 fuse_steps=True):
+                 steps_per_launch=None):
         if not torch.cuda.is_available():
             raise RuntimeError("Simulation needs a GPU")
         self.s = setup
@@ -68,9 +68,18 @@ class Simulation:
         self.N = N
         self.steps_done = 0
         self._step = lib().pf_step_f64 if dtype == torch.float64 else lib().pf_step_f32
-        self._step2 = lib().pf_step2_f64 if dtype == torch.float64 else lib().pf_step2_f32
+        f64 = dtype == torch.float64
+        self._stepk = {2: lib().pf_step2_f64 if f64 else lib().pf_step2_f32,
+                       3: lib().pf_step3_f64 if f64 else lib().pf_step3_f32}
+        # three steps per launch: 1/3 of the one-step HBM traffic per step
+        if steps_per_launch is None:
+            steps_per_launch = 3
+        if steps_per_launch not in (1, 2, 3):
+            raise ValueError("steps_per_launch must be 1, 2 or 3")
         five_point = all(setup.mask[k] == 0.0 for k in (0, 2, 6, 8))
-        self.fuse_steps = fuse_steps and five_point and self.nx >= 5 and self.ny >= 5
+        if not (five_point and self.nx >= 5 and self.ny >= 5):
+            steps_per_launch = 1
+        self.steps_per_launch = steps_per_launch
         self._summ = lib().pf_summarize_f64 if dtype == torch.float64 else lib().pf_summarize_f32
 
     def set_tile(self, rows, remap=1, depth=0):
@@ -78,6 +87,9 @@ class Simulation:
 
     def set_tile2(self, rows, depth=0):
         check(lib().pf_ctx_set_tile2(self.ctx, rows, depth), "pf_ctx_set_tile2")
+
+    def set_tile3(self, rows, depth=0):
+        check(lib().pf_ctx_set_tile3(self.ctx, rows, depth), "pf_ctx_set_tile3")
 
     def _nuc_due(self, t):
         return self.nucleation and t % self.nuc_every == 0
@@ -88,13 +100,18 @@ class Simulation:
         while left > 0:
             a, b = self.a, self.b
             t = self.steps_done
-            if (self.fuse_steps and left >= 2 and not self._nuc_due(t)
-                    and not self._nuc_due(t + 1)):
-                check(self._step2(self.ctx, self.nx, self.ny, self.s.bc, ctypes.byref(self.P),
-                                  _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(a[3]),
-                                  _ptr(b[0]), _ptr(b[1]), _ptr(b[2]), _ptr(b[3]), st),
-                      "pf_step2")
-                done = 2
+            k = 1
+            for cand in (3, 2):
+                if (cand <= self.steps_per_launch and left >= cand
+                        and not any(self._nuc_due(t + j) for j in range(cand))):
+                    k = cand
+                    break
+            if k > 1:
+                check(self._stepk[k](self.ctx, self.nx, self.ny, self.s.bc, ctypes.byref(self.P),
+                                     _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(a[3]),
+                                     _ptr(b[0]), _ptr(b[1]), _ptr(b[2]), _ptr(b[3]), st),
+                      "pf_step%d" % k)
+                done = k
             else:
                 nuc = None
                 if self._nuc_due(t):

@@ -44,7 +44,7 @@ def plain(setup, fields, steps, dtype, **kw):
 JOBS = [(64, 64, 1), (130, 71, 0), (37, 200, 1), (200, 33, 0), (1, 5, 1), (63, 62, 0)]
 
 
-@pytest.mark.parametrize("launch_steps", [1, 2])
+@pytest.mark.parametrize("launch_steps", [1, 2, 3])
 @pytest.mark.parametrize("storage", [PF_STORAGE_F64, PF_STORAGE_F32])
 def test_batch_equals_plain_alone(gpu, storage, launch_steps):
     """Each job (odd and even targets, tiny and ragged grids) against the
@@ -65,14 +65,14 @@ def test_batch_equals_plain_alone(gpu, storage, launch_steps):
         steps, active = b.poll(jid)
         assert steps == n and not active
         got = b.read(jid)
-        for fuse in (False, True):
-            want = plain(s, f, n, dtype, fuse_steps=fuse)
+        for spl in (1, 2, 3):
+            want = plain(s, f, n, dtype, steps_per_launch=spl)
             for k in range(4):
-                assert np.array_equal(got[k].astype(np.float64), want[k]), (jid, fuse, k)
+                assert np.array_equal(got[k].astype(np.float64), want[k]), (jid, spl, k)
     b.close()
 
 
-@pytest.mark.parametrize("launch_steps", [1, 2])
+@pytest.mark.parametrize("launch_steps", [1, 2, 3])
 def test_alone_vs_mix_graph_and_eager(gpu, launch_steps):
     """FP32 job X with nucleation on its schedule: alone (eager) == in a mix
     (graph replays, admissions and a removal in between, other jobs at other
@@ -118,12 +118,12 @@ def test_graph_replays_match_eager(gpu):
     s = M.make_setup(120, 70, bc=0)
     f = rand_fields(120, 70, 0, 11)
     res = []
-    for spg, ls in ((0, 1), (4, 1), (0, 2), (4, 2)):
+    for spg, ls in ((0, 1), (4, 1), (0, 2), (4, 2), (0, 3), (4, 3)):
         b = Batch(PF_STORAGE_F64, pool_bytes=64 << 20)
         b.set_launch_steps(ls)
         j = b.add(make_desc(s, 80), f)
         for r in range(5):                 # eager host work between replays
-            b.run(8 * (3 - ls), spg)
+            b.run(16 // ls + 1, spg)
             b.poll(j)
         assert b.poll(j) == (80, False)
         res.append(b.read(j))
@@ -133,7 +133,7 @@ def test_graph_replays_match_eager(gpu):
             assert np.array_equal(res[0][k], r[k])
 
 
-@pytest.mark.parametrize("launch_steps", [1, 2])
+@pytest.mark.parametrize("launch_steps", [1, 2, 3])
 def test_batch_nucleation_matches_plain(gpu, launch_steps):
     """Nucleation schedule, random stream and embeds: identical to the plain
     fused path, with P inside (0, 1)."""

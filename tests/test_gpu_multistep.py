@@ -1,9 +1,11 @@
-"""The two-step (temporally blocked) kernel, pf_step2_*, through the C-ABI.
+"""The temporally blocked kernels, pf_step2_* and pf_step3_*, through the C-ABI.
 
-It must give what two one-step launches give: the same interior values and
-the same ghost-ring images, for both boundary modes, both storage types,
-ragged grids and every tile shape.  Its parity with the oracle over long runs
-is covered by the Simulation-driven tests, which pair steps through it.
+Each must give what two (three) one-step launches give: the same interior
+values and the same ghost-ring images, for both boundary modes, both storage
+types, ragged grids (strips and chunks that do not divide the grid, chunks of
+one row, tiny grids where the images sit inside the first chunk) and every
+tile shape.  Their parity with the oracle over long runs is covered by the
+Simulation-driven tests, which group steps through them.
 """
 import ctypes
 
@@ -64,52 +66,60 @@ SHAPES = [(3, 3), (4, 5), (5, 4), (59, 7), (60, 61), (61, 62), (62, 3), (120, 9)
           (200, 131), (37, 300)]
 
 
+@pytest.mark.parametrize("k", [2, 3])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("bc", [0, 1])
 @pytest.mark.parametrize("shape", SHAPES)
-def test_two_steps_in_one_launch(ctx, shape, bc, dtype):
+def test_k_steps_in_one_launch(ctx, shape, bc, dtype, k):
     Nx, Ny = shape
     nx, ny = Nx + 2, Ny + 2
     s = M.make_setup(Nx, Ny, bc=bc, dt_rule=4e-7)
     P = _params(s)
     f0 = [torch.as_tensor(f).to(device="cuda", dtype=dtype) for f in
           _fields(Nx, Ny, bc, 1000 * Nx + Ny)]
-    mid = [torch.full_like(x, np.nan) for x in f0]
-    ref = [torch.full_like(x, np.nan) for x in f0]
     one = lib().pf_step_f64 if dtype == torch.float64 else lib().pf_step_f32
-    two = lib().pf_step2_f64 if dtype == torch.float64 else lib().pf_step2_f32
-    check(one(ctx, nx, ny, bc, ctypes.byref(P), *map(_p, f0), *map(_p, mid), None, None), "step")
-    check(one(ctx, nx, ny, bc, ctypes.byref(P), *map(_p, mid), *map(_p, ref), None, None), "step")
-    want = [x.double().cpu().numpy() for x in ref]
+    f64 = dtype == torch.float64
+    multi = {2: lib().pf_step2_f64 if f64 else lib().pf_step2_f32,
+             3: lib().pf_step3_f64 if f64 else lib().pf_step3_f32}[k]
+    set_tile = {2: lib().pf_ctx_set_tile2, 3: lib().pf_ctx_set_tile3}[k]
+    cur = f0
+    for _ in range(k):
+        nxt = [torch.full_like(x, np.nan) for x in f0]
+        check(one(ctx, nx, ny, bc, ctypes.byref(P), *map(_p, cur), *map(_p, nxt), None, None),
+              "step")
+        cur = nxt
+    want = [x.double().cpu().numpy() for x in cur]
     outs = []
-    for rows, depth in ((0, 0), (1, 2), (5, 6), (6, 3), (7, 2), (64, 3)):
-        check(lib().pf_ctx_set_tile2(ctx, rows, depth), "tile2")
+    for rows, depth in ((0, 0), (1, 2), (2, 3), (5, 6), (6, 3), (7, 2), (64, 3)):
+        check(set_tile(ctx, rows, depth), "tile")
         out = [torch.full_like(x, np.nan) for x in f0]
-        check(two(ctx, nx, ny, bc, ctypes.byref(P), *map(_p, f0), *map(_p, out), None), "step2")
+        check(multi(ctx, nx, ny, bc, ctypes.byref(P), *map(_p, f0), *map(_p, out), None),
+              "step%d" % k)
         got = [x.double().cpu().numpy() for x in out]
-        for k in range(4):
+        for f in range(4):
             # interior and ghost ring alike
-            assert np.all(np.isfinite(got[k])), (rows, depth, k)
-            assert _mixed_err(got[k], want[k]) <= (TOL if dtype == torch.float64 else 0.0), \
-                (rows, depth, k, _mixed_err(got[k], want[k]))
+            assert np.all(np.isfinite(got[f])), (rows, depth, f)
+            assert _mixed_err(got[f], want[f]) <= (TOL if f64 else 0.0), \
+                (rows, depth, f, _mixed_err(got[f], want[f]))
         outs.append(got)
-    check(lib().pf_ctx_set_tile2(ctx, 0, 0), "tile2")
+    check(set_tile(ctx, 0, 0), "tile")
     for o in outs[1:]:
-        for k in range(4):
-            assert np.array_equal(o[k], outs[0][k])
+        for f in range(4):
+            assert np.array_equal(o[f], outs[0][f])
 
 
 @pytest.mark.parametrize("bc", [0, 1])
-def test_fused_run_matches_one_step_run(gpu, oracle, bc):
-    """A Simulation pairing steps through the two-step kernel, with the
-    reference's nucleation schedule (odd interval, so pairs straddle the
+@pytest.mark.parametrize("spl", [2, 3])
+def test_fused_run_matches_one_step_run(gpu, oracle, bc, spl):
+    """A Simulation grouping steps through the multi-step kernels, with the
+    reference's nucleation schedule (odd interval, so groups straddle the
     nucleation steps), against one launch per step and against the oracle."""
     Nx, Ny = 150, 97
     s = M.make_setup(Nx, Ny, bc=bc)
     f0 = M.init_matrix_with_particles(Nx, Ny, s.dx, bc, seed=3, radius=2.5e-9)
-    a = Simulation(s, f0, nucleation=True, nuc_every=7)
-    b = Simulation(s, f0, nucleation=True, nuc_every=7, fuse_steps=False)
-    assert a.fuse_steps and not b.fuse_steps
+    a = Simulation(s, f0, nucleation=True, nuc_every=7, steps_per_launch=spl)
+    b = Simulation(s, f0, nucleation=True, nuc_every=7, steps_per_launch=1)
+    assert a.steps_per_launch == spl and b.steps_per_launch == 1
     for n in (1, 2, 5, 40, 153):
         a.step(n)
         b.step(n)
@@ -123,11 +133,12 @@ def test_fused_run_matches_one_step_run(gpu, oracle, bc):
     b.close()
 
 
-def test_step2_errors(ctx):
+@pytest.mark.parametrize("k", [2, 3])
+def test_multistep_errors(ctx, k):
     s = M.make_setup(8, 8, bc=0)
     P = _params(s)
     f = [torch.zeros(10 * 10, dtype=torch.float64, device="cuda") for _ in range(8)]
-    two = lib().pf_step2_f64
+    two = {2: lib().pf_step2_f64, 3: lib().pf_step3_f64}[k]
     # in place
     assert two(ctx, 10, 10, 0, ctypes.byref(P), *map(_p, f[:4]), *map(_p, f[:4]), None) != 0
     # nine-point mask
