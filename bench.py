@@ -19,6 +19,9 @@ cpu_baseline: the FP64 CPU oracle (OpenMP, all host threads) on a bounded
 sample of the same workload.
 
 Also reported (same JSON line):
+  dropin   -- the reference driver's per-step call sequence through the C++
+              drop-in API (tests/cpp/replay_main) on the same grid (no-flux,
+              nucleation on, no I/O), grouped three steps per launch and one.
   batched  -- 512 independent simulations, square sides log-uniform in
               128..1024, FP32 storage with FP64 arithmetic and reductions, one
               persistent launch per one, two or three steps replayed from a
@@ -287,6 +290,50 @@ def batched_section(args):
     }
 
 
+def dropin_section(args):
+    """The reference driver's call sequence (src/main.cpp:289-321) through the
+    C++ drop-in API (tests/cpp/replay_main, which replays it call for call) on
+    the plain workload: host wall clock over the last two thirds of the steps,
+    until the device has finished them (the final read-out to host memory is
+    reported on its own)."""
+    import re
+    import subprocess
+    import tempfile
+
+    import numpy as np
+
+    from pfpa import model as M
+    harness = os.path.join(ROOT, "tests", "cpp", "replay_main")
+    if not os.path.exists(harness):
+        return None
+    N = args.size
+    s = M.make_setup(N, N, bc=0)
+    f0 = M.init_enriched_band(N, N, 0, seed=42)
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        with open(fin, "wb") as fh:
+            fh.write(np.array([N + 2, N + 2], dtype=np.int32).tobytes())
+            for a in f0:
+                fh.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+        del f0
+        nuc = M.nuc_interval(s.dt)
+        for spl in (3, 1):
+            r = subprocess.run([harness, fin, os.path.join(td, "out.bin"), str(args.dropin_steps),
+                                repr(s.dt), repr(s.dx), str(nuc), "42", "0", str(spl)],
+                               capture_output=True, text=True, timeout=600)
+            if r.returncode != 0:
+                raise RuntimeError("drop-in replay failed: " + r.stderr[-2000:])
+            m = re.search(r"last (\d+) steps in ([0-9.eE+-]+) s; read-out ([0-9.eE+-]+) s",
+                          r.stdout)
+            n, secs, rsecs = int(m.group(1)), float(m.group(2)), float(m.group(3))
+            out["steps_per_launch_%d" % spl] = {"value": N * N * n / secs,
+                                                "ms_per_step": secs / n * 1e3, "steps_timed": n,
+                                                "read_out_s": rsecs}
+    return {"unit": "cell-updates/s", "grid": [N, N], "bc": "no-flux (the driver's)",
+            "nucleation_interval": nuc, **out}
+
+
 def serving_section(args):
     from pfpa.serve import fp64_replay_error, make_requests, serve
     reqs = make_requests(args.serve_rate, args.serve_duration, seed=1)
@@ -347,6 +394,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-only", action="store_true")
+    ap.add_argument("--dropin-steps", type=int, default=900,
+                    help="steps of the C++ drop-in replay (the driver's call sequence)")
     ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=5000)
     ap.add_argument("--loop-steps", type=int, default=20,
@@ -416,6 +465,8 @@ def main():
         "roofline": plain["roofline"],
     }
     if not args.plain_only:
+        out["dropin"] = dropin_section(args)
+        log("dropin:", json.dumps(out["dropin"]))
         out["batched"] = batched_section(args)
         log("batched:", json.dumps(out["batched"]))
         out["serving"] = serving_section(args)

@@ -60,4 +60,17 @@ void device_set_boundary(struct CudaData* dev, int bc);
 /** Seed of the nucleation stream (default 42); resets the stream position. */
 void device_seed_prng(struct CudaData* dev, uint64_t seed);
 
+/** Steps per launch of device_evolution: 3 (default) or 1.  With 3, calls to
+ * device_evolution are grouped: three consecutive steps (the driver swapping
+ * the old/new pointers in between, src/main.cpp:318-321) run as one launch of
+ * the three-step kernel, reading the first step's old fields and writing the
+ * third step's new fields.  Every function of this API that reads the fields
+ * (device_nucleation, device_dataviz, read_out_result, free_cuda) first runs
+ * the pending steps one at a time, so what the driver observes is unchanged.
+ * Code that reads the device arrays by other means calls device_flush first. */
+void device_set_steps_per_launch(struct CudaData* dev, int steps);
+
+/** Run the steps device_evolution has deferred (see above). */
+void device_flush(struct CudaData* dev);
+
 #endif /* _CUDA_KERNELS_H_ */

@@ -83,6 +83,13 @@ struct CudaData {
 	fp_t DCr[NC], DNb[NC];
 	fp_t kappa[NP], omega[NP], Lmob[NP];
 	fp_t mask[9];
+	/* grouped launches: steps issued by device_evolution and not yet run */
+	int group;           /* steps per launch (1 or 3) */
+	int pend;            /* pending steps (0..2) */
+	fp_t* pend_old[4];   /* old and new fields of the first pending step */
+	fp_t* pend_new[4];
+	fp_t pend_alpha, pend_dt;
+	int pend_nx, pend_ny;
 };
 
 #endif /* _TYPE_H_ */

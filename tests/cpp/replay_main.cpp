@@ -6,13 +6,18 @@
 // swap_pointers_1D x4, and on I/O steps device_dataviz + read_out_result.
 //
 //   replay_main <in.bin> <out.bin> <steps> <dt> <dx> <nuc_interval> <seed> <io_interval>
+//               [steps_per_launch]
 //
 // in.bin / out.bin: int32 nx, int32 ny, then conc_Cr, conc_Nb, phi_del,
 // phi_lav as nx*ny doubles each (ghost ring included).  nuc_interval 0 = off.
+// steps_per_launch (1 or 3, default: the library's) is the drop-in's grouping
+// of device_evolution calls.
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
+
+#include <chrono>
 
 #include "data.cuh"
 #include "discretization.cuh"
@@ -20,10 +25,14 @@
 #include "pfpa.h"
 
 int main(int argc, char* argv[]) {
-  if (argc != 9) {
-    fprintf(stderr, "usage: %s in.bin out.bin steps dt dx nuc_interval seed io_interval\n", argv[0]);
+  if (argc != 9 && argc != 10) {
+    fprintf(stderr,
+            "usage: %s in.bin out.bin steps dt dx nuc_interval seed io_interval "
+            "[steps_per_launch]\n",
+            argv[0]);
     return 2;
   }
+  const int steps_per_launch = argc == 10 ? atoi(argv[9]) : 0;
   const long steps = atol(argv[3]);
   const double dt = atof(argv[4]);
   const double meshres = atof(argv[5]);
@@ -66,9 +75,21 @@ int main(int argc, char* argv[]) {
   init_cuda(&host, nx, ny, nm, kappa, omega, Lmob, D_Cr, D_Nb, &dev);
   device_init_prng(&dev, nx, ny, nm, bx, by);
   device_seed_prng(&dev, seed);
+  if (steps_per_launch) device_set_steps_per_launch(&dev, steps_per_launch);
 
+  // the last two thirds of the steps are timed (host wall clock from a point
+  // where the device has finished the first third until it has finished all),
+  // so that code loading on the first launches is excluded; the final read-out
+  // is timed on its own
   long io_steps = 0;
+  const long t_from = steps / 3;
+  auto t0 = std::chrono::steady_clock::now();
   for (long kernel_time = 0; kernel_time < steps; kernel_time++) {
+    if (kernel_time == t_from) {
+      device_flush(&dev);
+      (void)hipDeviceSynchronize();
+      t0 = std::chrono::steady_clock::now();
+    }
     device_boundaries(&dev, nx, ny, nm, bx, by);
     device_fictitious(&dev, nx, ny, nm, bx, by);
     fictitious_boundaries(&dev, nx, ny, nm, bx, by);
@@ -88,7 +109,13 @@ int main(int argc, char* argv[]) {
       io_steps++;
     }
   }
+  device_flush(&dev);
+  (void)hipDeviceSynchronize();
+  const auto t1 = std::chrono::steady_clock::now();
+  const double secs = std::chrono::duration<double>(t1 - t0).count();
   read_out_result(&dev, &host, nx, ny);
+  const double read_secs =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
 
   FILE* fo = fopen(argv[2], "wb");
   if (!fo) return 4;
@@ -101,6 +128,7 @@ int main(int argc, char* argv[]) {
   fclose(fo);
   free_cuda(&dev);
   free_arrays(&host);
-  printf("replayed %ld steps, %ld I/O steps\n", steps, io_steps);
+  printf("replayed %ld steps, %ld I/O steps; last %ld steps in %.6f s; read-out %.6f s\n",
+         steps, io_steps, steps - t_from, secs, read_secs);
   return 0;
 }

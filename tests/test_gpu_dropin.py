@@ -32,12 +32,8 @@ def oracle_reference(oracle, f0, setup, steps):
     return oracle.run(f0, P, 0, steps)
 
 
-def test_cpp_driver_replay(gpu, oracle):
-    assert os.path.exists(HARNESS), "tests/cpp/replay_main not built: run __graft_entry__.build()"
-    Nx, Ny = 120, 90
-    s = M.make_setup(Nx, Ny, bc=0)
-    f0 = M.init_matrix_with_particles(Nx, Ny, s.dx, 0, seed=5, radius=2.5e-9)
-    steps = 300
+def run_harness(f0, s, steps, nuc, io, spl=None):
+    Nx, Ny = s.N
     with tempfile.TemporaryDirectory() as td:
         fin = os.path.join(td, "in.bin")
         fout = os.path.join(td, "out.bin")
@@ -45,20 +41,48 @@ def test_cpp_driver_replay(gpu, oracle):
             fh.write(struct.pack("ii", Nx + 2, Ny + 2))
             for a in f0:
                 fh.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
-        r = subprocess.run([HARNESS, fin, fout, str(steps), repr(s.dt), repr(s.dx), "100", "42",
-                            "150"], capture_output=True, text=True, timeout=120)
+        args = [HARNESS, fin, fout, str(steps), repr(s.dt), repr(s.dx), str(nuc), "42", str(io)]
+        if spl is not None:
+            args.append(str(spl))
+        r = subprocess.run(args, capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stderr
         raw = np.fromfile(fout, dtype=np.uint8)
-    n = (Nx + 2) * (Ny + 2)
-    arr = raw[8:].view(np.float64).reshape(6, Ny + 2, Nx + 2)
+    return raw[8:].view(np.float64).reshape(6, Ny + 2, Nx + 2)
+
+
+def test_cpp_driver_replay(gpu, oracle):
+    """The reference's call sequence with nucleation every 100 steps and I/O
+    every 150: grouped (three steps per launch, the default) and one launch per
+    call give the same bits, and both match the oracle."""
+    assert os.path.exists(HARNESS), "tests/cpp/replay_main not built: run __graft_entry__.build()"
+    Nx, Ny = 120, 90
+    s = M.make_setup(Nx, Ny, bc=0)
+    f0 = M.init_matrix_with_particles(Nx, Ny, s.dx, 0, seed=5, radius=2.5e-9)
+    steps = 300
+    arr = run_harness(f0, s, steps, 100, 150)
+    one = run_harness(f0, s, steps, 100, 150, spl=1)
     want = oracle_reference(oracle, f0, s, steps)
     for k in range(4):
+        assert np.array_equal(arr[k], one[k]), k
         assert mixed_err(arr[k], want[k]) <= TOL, k
     # visualisation fields were produced at the last I/O step (step 300)
+    assert np.array_equal(arr[4:], one[4:])
     assert np.allclose(arr[4], 1.0 - arr[0] - arr[1], atol=1e-15)
     p = lambda x: x ** 3 * (6 * x * x - 15 * x + 10)  # noqa: E731
     assert np.allclose(arr[5], p(arr[2]) + p(arr[3]), rtol=1e-13, atol=1e-15)
-    assert n == arr[0].size
+    assert (Nx + 2) * (Ny + 2) == arr[0].size
+
+
+@pytest.mark.parametrize("io", [1, 2, 7])
+def test_cpp_driver_replay_frequent_io(gpu, io):
+    """I/O (device_dataviz + read_out_result) every 1, 2 or 7 steps breaks the
+    groups at every phase; results equal one launch per call."""
+    Nx, Ny = 80, 48
+    s = M.make_setup(Nx, Ny, bc=1)
+    f0 = M.init_matrix_with_particles(Nx, Ny, s.dx, 1, seed=9, radius=1.2e-9)
+    a = run_harness(f0, s, 23, 5, io)      # 23 steps: a pending tail flushed by read-out
+    b = run_harness(f0, s, 23, 5, io, spl=1)
+    assert np.array_equal(a, b)
 
 
 def test_python_api_replay(gpu, oracle):

@@ -55,7 +55,10 @@ def test_exports_dropin_cpp_api():
                  "device_evolution(CudaData*", "device_nucleation(CudaData*",
                  "device_dataviz(CudaData*", "make_arrays(HostData*", "free_arrays(HostData*",
                  "swap_pointers_1D(double**", "five_point_Laplacian_stencil(double",
-                 "checkCuda(hipError_t", "nTiles(int"):
+                 "checkCuda(hipError_t", "nTiles(int",
+                 # additions: boundary mode, seed, grouped launches
+                 "device_set_boundary(CudaData*", "device_seed_prng(CudaData*",
+                 "device_set_steps_per_launch(CudaData*", "device_flush(CudaData*"):
         assert name in out, name
 
 
