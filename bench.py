@@ -66,6 +66,19 @@ def load_traffic(workload):
     return d.get("hbm_bytes_per_launch")
 
 
+def load_valu_issue(kernel):
+    """VALU issue time per launch (us) of `kernel` at 4096^2 from the committed
+    rocprofv3 SQ counter summary (profiles/sq_counters_steps_4096.txt), or None."""
+    path = os.path.join(ROOT, "profiles", "sq_counters_steps_4096.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith(kernel):
+                return float(line.split()[-1])
+    return None
+
+
 def cpu_baseline(N, steps, warmup=1):
     """Time the FP64 oracle (all host threads via OpenMP) on N x N for `steps`."""
     import oracle as O
@@ -159,6 +172,8 @@ def plain_section(args, dtype_name="f64"):
 
     nuc_every = sim.nuc_every
     sim.close()
+    valu_us = load_valu_issue("pf::stepk_kernel<%s, 1, 3, 3>" % ("double" if f64 else "float")) \
+        if N == 4096 else None
     return {
         "value": value, "ms_per_step": ms / args.steps, "N": N, "nuc_every": nuc_every,
         "dt": s.dt,
@@ -171,6 +186,12 @@ def plain_section(args, dtype_name="f64"):
             "algorithmic_bytes_per_launch": alg_bytes,
             "cell_updates_per_launch": 3 * cells,
             "algorithmic_bytes_per_cell_update": bpc / 3,
+            # the rate per step against the one-step roofline (bpc bytes per
+            # cell-update at peak HBM), and the share of the launch the VALU
+            # instructions alone need (rocprofv3 SQ_INSTS_VALU, profiles/)
+            "cell_update_rate_vs_one_step_roofline": value * bpc / HBM_PEAK,
+            "valu_issue_us": valu_us,
+            "valu_issue_frac": (valu_us / (kern3_s * 1e6)) if valu_us else None,
             "two_step_kernel": other("pf::stepk_kernel<K=2> (two fused steps per launch)", 2,
                                      kern2_s),
             "one_step_kernel": other("pf::step_kernel (one fused step per launch)", 1, kern1_s),
@@ -406,6 +427,9 @@ def main():
     ap.add_argument("--serve-lpg", type=int, default=17,
                     help="step launches per captured graph in the serving loop")
     ap.add_argument("--precision-marks", type=int, nargs="+", default=[1000, 10000, 20000])
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only launch the one-, two- and three-step kernels (FP64, FP32) at --size "
+                         "a few times each, for rocprofv3 counter runs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain device copy of 4 x 4098^2 FP64 arrays (1.07 GB moved per "
                          "dispatch) to scale rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -428,6 +452,27 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibration": "torch copy_ of 4 x %d FP64" % n,
                           "bytes_read_per_dispatch": n * 8, "bytes_written_per_dispatch": n * 8}))
+        return
+
+    if args.kernels_only:
+        # one-, two- and three-step launches at --size, FP64 then FP32, for
+        # counter collection (rocprofv3 --pmc); prints one JSON line
+        import torch
+
+        from pfpa import model as M
+        from pfpa.sim import Simulation
+        N = args.size or 4096
+        s = M.make_setup(N, N, bc=1)
+        f0 = M.init_enriched_band(N, N, 1, seed=42)
+        for dt in (torch.float64, torch.float32):
+            sim = Simulation(s, f0, dtype=dt)
+            for k, n in ((1, 1), (2, 20), (3, 21)):
+                sim.steps_per_launch = k
+                sim.step(n)
+            torch.cuda.synchronize()
+            sim.close()
+        print(json.dumps({"kernels_only": [N, N], "launches": {"one_step": 1, "two_step": 10,
+                                                               "three_step": 7}}))
         return
 
     if args.form == "precision":

@@ -15,9 +15,15 @@ runs in libpfpa.so.  Errors raise PfError (the reference asserts in checkCuda,
 src/discretization.cu:23-30).
 
 Differences from the reference, by design:
-  * device_evolution launches one fused kernel that performs the whole step
+  * device_evolution runs one fused kernel that performs the whole step
     (boundaries, fictitious compositions, Laplacians, Cahn-Hilliard and
     Allen-Cahn); the five calls before it only record that they were made.
+  * Calls to device_evolution are grouped three at a time (as in the C++
+    drop-in): three consecutive steps, the caller swapping the old/new
+    fields in between, run as one launch of the three-step kernel.  Every
+    function here that reads the fields first runs the pending steps one at a
+    time, so what the caller observes is unchanged; device_flush runs them
+    explicitly and device_set_steps_per_launch(dev, 1) turns grouping off.
   * The per-cell cuRAND state (seeded from clock()) is replaced by a
     counter-based Philox4x32-10 stream: device_init_prng(seed=...) sets the
     key, and each device_nucleation call advances the stream position.
@@ -74,6 +80,9 @@ class CudaData:
         self.seed = DEFAULT_SEED
         self.nuc_calls = 0
         self.pending = []
+        self.group = 3        # steps per launch of grouped device_evolution calls
+        self.queued = 0       # steps recorded and not yet run (0..2)
+        self.first = None     # (old, new, alpha, dt, nx, ny) of the first of them
 
 
 def make_arrays(host, nx, ny, nm):
@@ -143,6 +152,7 @@ def init_cuda(host, nx, ny, nm, kappa, omega, Lmob, D_Cr, D_Nb, dev, bc=_lib.PF_
 def free_cuda(dev):
     """src/data.cu:94-154."""
     if dev.ctx is not None:
+        device_flush(dev)
         torch.cuda.synchronize()
         lib().pf_ctx_destroy(dev.ctx)
         dev.ctx = None
@@ -188,18 +198,65 @@ def device_laplacian_boundaries(dev, nx, ny, nm, bx, by):
     _record(dev, "laplacian_boundaries")
 
 
-def device_evolution(dev, nx, ny, nm, bx, by, alpha, dt):
-    """src/discretization.cu:982-1020: one fused step *_old -> *_new."""
-    dev.pending.clear()
+_FIELDS = ("conc_Cr", "conc_Nb", "phi_del", "phi_lav")
+
+
+def _fields(dev, which):
+    return tuple(getattr(dev, f + which) for f in _FIELDS)
+
+
+def _launch(dev, fn, nx, ny, alpha, dt, old, new, *extra):
     P = dev.params
     P.alpha = alpha
     P.dt = dt
-    check(lib().pf_step_f64(dev.ctx, nx, ny, dev.bc, ctypes.byref(P),
-                            _ptr(dev.conc_Cr_old), _ptr(dev.conc_Nb_old),
-                            _ptr(dev.phi_del_old), _ptr(dev.phi_lav_old),
-                            _ptr(dev.conc_Cr_new), _ptr(dev.conc_Nb_new),
-                            _ptr(dev.phi_del_new), _ptr(dev.phi_lav_new),
-                            None, _stream()), "device_evolution")
+    check(fn(dev.ctx, nx, ny, dev.bc, ctypes.byref(P), *map(_ptr, old), *map(_ptr, new),
+             *extra, _stream()), "device_evolution")
+
+
+def device_flush(dev):
+    """Run the steps device_evolution has recorded and not yet launched, one
+    at a time: first old -> new, then back."""
+    if dev.queued:
+        old, new, alpha, dt, nx, ny = dev.first
+        for i in range(dev.queued):
+            a, b = (old, new) if i % 2 == 0 else (new, old)
+            _launch(dev, lib().pf_step_f64, nx, ny, alpha, dt, a, b, None)
+    dev.queued = 0
+    dev.first = None
+
+
+def device_set_steps_per_launch(dev, steps):
+    """3 (default): group device_evolution calls three per launch; 1: one."""
+    device_flush(dev)
+    dev.group = 3 if steps == 3 else 1
+
+
+def device_evolution(dev, nx, ny, nm, bx, by, alpha, dt):
+    """src/discretization.cu:982-1020: one fused step *_old -> *_new, grouped
+    three to a launch (see the module docstring)."""
+    dev.pending.clear()
+    old, new = _fields(dev, "_old"), _fields(dev, "_new")
+    m = dev.params.mask
+    five_point = m[0] == 0.0 and m[2] == 0.0 and m[6] == 0.0 and m[8] == 0.0
+    if dev.group != 3 or not five_point or nx < 5 or ny < 5:
+        device_flush(dev)
+        _launch(dev, lib().pf_step_f64, nx, ny, alpha, dt, old, new, None)
+        return
+    if dev.queued:
+        f_old, f_new, f_alpha, f_dt, f_nx, f_ny = dev.first
+        want_old, want_new = (f_new, f_old) if dev.queued % 2 else (f_old, f_new)
+        same = (lambda a, b: all(x.data_ptr() == y.data_ptr() for x, y in zip(a, b)))
+        if not (same(old, want_old) and same(new, want_new) and (alpha, dt, nx, ny) ==
+                (f_alpha, f_dt, f_nx, f_ny)):
+            device_flush(dev)
+    if not dev.queued:
+        dev.first = (old, new, alpha, dt, nx, ny)
+    dev.queued += 1
+    if dev.queued == 3:
+        f_old, f_new = dev.first[0], dev.first[1]
+        _launch(dev, lib().pf_step3_f64, nx, ny, alpha, dt, f_old, f_new)
+        dev.queued = 0
+        dev.first = None
 
 
 def nuc_params(dev, sigma_del, sigma_lav, unit_a, ifce_width, dx, dy, dz, dt):
@@ -215,6 +272,7 @@ def nuc_params(dev, sigma_del, sigma_lav, unit_a, ifce_width, dx, dy, dz, dt):
 def device_nucleation(dev, nx, ny, nm, bx, by, sigma_del, sigma_lav, unit_a, ifce_width,
                       dx, dy, dz, dt):
     """src/discretization.cu:1179-1209: nucleation pass on the *_new fields."""
+    device_flush(dev)
     N = nuc_params(dev, sigma_del, sigma_lav, unit_a, ifce_width, dx, dy, dz, dt)
     check(lib().pf_nucleation_f64(dev.ctx, nx, ny, _ptr(dev.conc_Cr_new), _ptr(dev.conc_Nb_new),
                                   _ptr(dev.phi_del_new), _ptr(dev.phi_lav_new),
@@ -224,6 +282,7 @@ def device_nucleation(dev, nx, ny, nm, bx, by, sigma_del, sigma_lav, unit_a, ifc
 
 def read_out_result(dev, host, nx, ny):
     """src/data.cu:156-172: device *_old -> host *_new (synchronous)."""
+    device_flush(dev)
     torch.cuda.synchronize()
     for name in ("conc_Cr", "conc_Nb", "phi_del", "phi_lav"):
         getattr(host, name + "_new")[...] = (
@@ -232,6 +291,7 @@ def read_out_result(dev, host, nx, ny):
 
 def device_dataviz(dev, host, nx, ny, nm, bx, by):
     """src/discretization.cu:1231-1263."""
+    device_flush(dev)
     check(lib().pf_dataviz_f64(nx, ny, _ptr(dev.conc_Cr_old), _ptr(dev.conc_Nb_old),
                                _ptr(dev.phi_del_old), _ptr(dev.phi_lav_old),
                                _ptr(dev.conc_Ni), _ptr(dev.phi), _stream()), "device_dataviz")
@@ -243,6 +303,7 @@ def device_dataviz(dev, host, nx, ny, nm, bx, by):
 def summarize_fields(dev, nx, ny):
     """Means of {x_Cr, x_Nb, gamma, delta, Laves} over the interior of the
     latest fields (src/alloy625.cpp:811-863), on the GPU."""
+    device_flush(dev)
     out = torch.zeros(5, dtype=torch.float64, device="cuda")
     check(lib().pf_summarize_f64(dev.ctx, nx, ny, _ptr(dev.conc_Cr_old), _ptr(dev.conc_Nb_old),
                                  _ptr(dev.phi_del_old), _ptr(dev.phi_lav_old), _ptr(out),

@@ -85,8 +85,10 @@ def test_cpp_driver_replay_frequent_io(gpu, io):
     assert np.array_equal(a, b)
 
 
-def test_python_api_replay(gpu, oracle):
-    """pfpa.api in the reference's call sequence, I/O steps included."""
+@pytest.mark.parametrize("spl", [3, 1])
+def test_python_api_replay(gpu, oracle, spl):
+    """pfpa.api in the reference's call sequence, I/O steps included, with
+    grouped launches (3, the default) and one launch per call."""
     Nx, Ny = 100, 70
     s = M.make_setup(Nx, Ny, bc=0)
     f0 = M.init_matrix_with_particles(Nx, Ny, s.dx, 0, seed=8, radius=2e-9)
@@ -102,6 +104,7 @@ def test_python_api_replay(gpu, oracle):
     D_Cr, D_Nb, Lmob = M.set_diffusion_matrix(0.30, 0.02)
     api.init_cuda(host, nx, ny, nm, M.kappa(), M.omega(), Lmob, D_Cr, D_Nb, dev)
     api.device_init_prng(dev, nx, ny, nm, bx, by, seed=42)
+    api.device_set_steps_per_launch(dev, spl)
     steps, nuc_interval, io_interval = 200, 50, 64
     for t in range(steps):
         api.device_boundaries(dev, nx, ny, nm, bx, by)
