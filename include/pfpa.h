@@ -142,6 +142,23 @@ int pf_summarize_f64(pf_ctx* ctx, int nx, int ny, const double* conc_Cr, const d
 int pf_summarize_f32(pf_ctx* ctx, int nx, int ny, const float* conc_Cr, const float* conc_Nb,
                      const float* phi_del, const float* phi_lav, double* out5, void* stream);
 
+/* Free energy of the fields: src/alloy625.cpp:752-784 (gibbs) + :786-806
+ * (maskedgradient) summed as in summarize_energy (:865-919), with the
+ * per-cell densities times dV = dx*dy; compensated FP64 sums in a fixed order.
+ * kappa, omega (2 each, delta then Laves) and Omega are the values of
+ * src/parameters.h:36-48; bc as for the step.  out3 (DEVICE, 3 doubles) =
+ * {total, chemical, gradient}; chem and grad (DEVICE nx*ny doubles, may be
+ * NULL) receive the per-cell energies at interior cells, as the driver's
+ * chem_nrg / grad_nrg host arrays do. */
+int pf_energy_f64(pf_ctx* ctx, int nx, int ny, int bc, double dx, double dy,
+                  const double* kappa, const double* omega, double Omega,
+                  const double* conc_Cr, const double* conc_Nb, const double* phi_del,
+                  const double* phi_lav, double* chem, double* grad, double* out3, void* stream);
+int pf_energy_f32(pf_ctx* ctx, int nx, int ny, int bc, double dx, double dy,
+                  const double* kappa, const double* omega, double Omega,
+                  const float* conc_Cr, const float* conc_Nb, const float* phi_del,
+                  const float* phi_lav, double* chem, double* grad, double* out3, void* stream);
+
 /* x_Ni = 1 - x_Cr - x_Nb and phi = p(phi_del) + p(phi_lav) over every cell.
  * Replaces dataviz_kernel (src/discretization.cu:1211-1229). */
 int pf_dataviz_f64(int nx, int ny, const double* conc_Cr, const double* conc_Nb,

@@ -87,6 +87,8 @@ def lib():
                                      ctypes.POINTER(ctypes.c_ubyte)])
         L.orc_summarize.restype = None
         L.orc_summarize.argtypes = [_i, _i] + [_dp] * 5
+        L.orc_energy.restype = None
+        L.orc_energy.argtypes = [_i, _i, _i, _d, _d, _dp, _dp, _d] + [_dp] * 7
         L.orc_num_threads.restype = _i
         L.orc_num_threads.argtypes = []
         _lib = L
@@ -283,3 +285,20 @@ def ref():
             getattr(L, name).argtypes = [_d] * 5
         _ref = L
     return _ref
+
+
+def energy(fields, bc, dx, dy, kappa, omega, Omega, maps=False):
+    """orc_energy: (total, sum chem, sum grad) of the free energy, and the
+    per-cell chem / grad arrays when `maps`."""
+    f = [np.ascontiguousarray(a, dtype=np.float64) for a in fields]
+    ny, nx = f[0].shape
+    chem = np.zeros((ny, nx)) if maps else None
+    grad = np.zeros((ny, nx)) if maps else None
+    k = np.ascontiguousarray(kappa, dtype=np.float64)
+    w = np.ascontiguousarray(omega, dtype=np.float64)
+    out = np.zeros(3)
+    lib().orc_energy(nx, ny, bc, dx, dy, _ptr(k), _ptr(w), Omega, *[_ptr(a) for a in f],
+                     _ptr(chem) if maps else None, _ptr(grad) if maps else None, _ptr(out))
+    if maps:
+        return tuple(out), chem, grad
+    return tuple(out)

@@ -586,6 +586,61 @@ void orc_summarize(int nx, int ny, const double *xCr, const double *xNb,
 		out5[k] = (double)(s[k] / N);
 }
 
+/* Free energy of the fields (src/alloy625.cpp:752-784 gibbs, :786-806
+ * maskedgradient, :865-919 summarize_energy), restated literally: per
+ * interior cell
+ *   chem = (pGam g_gam(x_gam) + pDel g_del(x_del) + pLav g_lav(x_lav)
+ *           + sum_i omega_i phi_i^2 (1 - phi_i)^2 + 2 Omega phi_del^2 phi_lav^2) dV
+ *   grad = sum_i kappa_i |grad phi_i|^2 dV,
+ * fictitious compositions from the KKS solve (orc_fict), gradients by central
+ * differences (phi(x+1) - phi(x-1)) / (2 dx) with the boundary map for the
+ * neighbours (no-flux: the MMSP Neumann ghost copy; periodic: wrap), dV = dx dy.
+ * chem[] and grad[] (nx*ny, may be NULL) receive the per-cell values at
+ * interior cells; out3 = {total, sum chem, sum grad} (long double sums). */
+void orc_energy(int nx, int ny, int bc, double dx, double dy, const double kappa[2],
+                const double omega[2], double Omega, const double *xCr, const double *xNb,
+                const double *pd, const double *pl, double *chem, double *grad, double out3[3])
+{
+	const int Nx = nx - 2, Ny = ny - 2;
+	const double dV = dx * dy;
+	long double sc = 0.0L, sg = 0.0L;
+	for (int y = 1; y <= Ny; y++) {
+		for (int x = 1; x <= Nx; x++) {
+			const size_t i = (size_t)nx * y + x;
+			const double pDel = orc_p(pd[i]), pLav = orc_p(pl[i]);
+			const double pGam = 1.0 - pDel - pLav;
+			double f[6];
+			orc_fict(xCr[i], xNb[i], pDel, pGam, pLav, f);
+			double g = pGam * orc_g(0, f[0], f[1]);
+			g += pDel * orc_g(1, f[2], f[3]);
+			g += pLav * orc_g(2, f[4], f[5]);
+			const double phi[2] = {pd[i], pl[i]};
+			for (int k = 0; k < 2; k++)
+				g += omega[k] * (phi[k] * phi[k]) * pow(1.0 - phi[k], 2);
+			g += 2.0 * Omega * (phi[0] * phi[0]) * (phi[1] * phi[1]);
+			const int xl = orc_bc(x - 1, Nx, bc), xh = orc_bc(x + 1, Nx, bc);
+			const int yl = orc_bc(y - 1, Ny, bc), yh = orc_bc(y + 1, Ny, bc);
+			const double *fl[2] = {pd, pl};
+			double ge = 0.0;
+			for (int k = 0; k < 2; k++) {
+				const double gxv = (fl[k][(size_t)nx * y + xh] - fl[k][(size_t)nx * y + xl]) / (2.0 * dx);
+				const double gyv = (fl[k][(size_t)nx * yh + x] - fl[k][(size_t)nx * yl + x]) / (2.0 * dy);
+				ge += kappa[k] * (gxv * gxv + gyv * gyv);
+			}
+			const double c = g * dV, gg = ge * dV;
+			if (chem)
+				chem[i] = c;
+			if (grad)
+				grad[i] = gg;
+			sc += c;
+			sg += gg;
+		}
+	}
+	out3[0] = (double)(sc + sg);
+	out3[1] = (double)sc;
+	out3[2] = (double)sg;
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

@@ -9,6 +9,7 @@
 #include "nucleation.cuh"
 #include "parabola625.cuh"
 #include "pf_internal.h"
+#include "pf_march.h"
 #include "pf_thermo.h"
 
 namespace pf {
@@ -27,7 +28,8 @@ hipError_t scratch_alloc(Scratch& s) {
   if (e != hipSuccess) return e;
   e = hipMalloc((void**)&s.events, (size_t)s.event_capacity * sizeof(Event));
   if (e != hipSuccess) return e;
-  e = hipMalloc((void**)&s.partials, (size_t)REDUCE_BLOCKS * 10 * sizeof(double));
+  // per-block partials of up to 5 compensated sums, then 8 doubles of final sums
+  e = hipMalloc((void**)&s.partials, ((size_t)REDUCE_BLOCKS * 10 + 8) * sizeof(double));
   return e;
 }
 void scratch_free(Scratch& s) {
@@ -270,16 +272,25 @@ __global__ __launch_bounds__(256) void summarize_partial_kernel(const T* __restr
     }
 }
 
-__global__ void summarize_final_kernel(const double* partials, int nblocks, double inv_n,
-                                       double* out5) {
-  const int k = threadIdx.x;
-  if (k >= 5) return;
+// Final pass over the per-block partials of `nq` quantities (stride 2*nq
+// doubles per block): block k merges quantity k, 64 threads over a fixed
+// partition, then a fixed tree.  out[k] = sum * scale.
+__global__ __launch_bounds__(64) void reduce_final_kernel(const double* partials, int nblocks,
+                                                          int nq, double scale, double* out) {
+  const int k = blockIdx.x, t = threadIdx.x;
   Kahan a{0.0, 0.0};
-  for (int b = 0; b < nblocks; b++) {
-    Kahan t{partials[(size_t)b * 10 + 2 * k], partials[(size_t)b * 10 + 2 * k + 1]};
-    kmerge(a, t);
+  for (int b = t; b < nblocks; b += 64) {
+    Kahan v{partials[(size_t)b * 2 * nq + 2 * k], partials[(size_t)b * 2 * nq + 2 * k + 1]};
+    kmerge(a, v);
   }
-  out5[k] = (a.s + a.c) * inv_n;
+  __shared__ Kahan sh[64];
+  sh[t] = a;
+  __syncthreads();
+  for (int w = 32; w > 0; w >>= 1) {
+    if (t < w) kmerge(sh[t], sh[t + w]);
+    __syncthreads();
+  }
+  if (t == 0) out[k] = (sh[0].s + sh[0].c) * scale;
 }
 
 template <typename T>
@@ -293,7 +304,7 @@ hipError_t launch_summarize(int nx, int ny, const T* xCr, const T* xNb, const T*
                      pl, nx, ny, S.partials);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(summarize_final_kernel, dim3(1), dim3(64), 0, st, S.partials, blocks,
+  hipLaunchKernelGGL(reduce_final_kernel, dim3(5), dim3(64), 0, st, S.partials, blocks, 5,
                      1.0 / (double)N, out5);
   return hipGetLastError();
 }
@@ -302,6 +313,110 @@ template hipError_t launch_summarize<double>(int, int, const double*, const doub
                                              const Scratch&, hipStream_t);
 template hipError_t launch_summarize<float>(int, int, const float*, const float*, const float*,
                                             const float*, double*, const Scratch&, hipStream_t);
+
+// ------------------------------------------------------------ free energy
+// src/alloy625.cpp:752-784 (gibbs), :786-806 (maskedgradient) and :865-919
+// (summarize_energy) per interior cell, times dV = dx dy:
+//   chem = sum_a p_a g_a(x_a) + sum_i omega_i phi_i^2 (1 - phi_i)^2
+//          + 2 Omega phi_del^2 phi_lav^2
+//   grad = sum_i kappa_i |grad phi_i|^2   (central differences)
+// With the KKS potential mu of the cell, x_a - xe_a = A_a mu and
+// g_a(x_a) = 1/2 mu.A_a.mu, so sum_a p_a g_a(x_a) = 1/2 mu.(M mu) = 1/2 mu.r,
+// r = x - sum_a p_a xe_a.  Neighbours of the gradients go through the
+// boundary map (no-flux: the MMSP Neumann ghost copy; periodic: wrap).
+template <typename T, int BC>
+__global__ __launch_bounds__(256) void energy_partial_kernel(
+    const T* __restrict__ xCr, const T* __restrict__ xNb, const T* __restrict__ pd,
+    const T* __restrict__ pl, int nx, int ny, EnergyParams E, double* __restrict__ chem_out,
+    double* __restrict__ grad_out, double* partials) {
+  const int Nx = nx - 2, Ny = ny - 2;
+  const long N = (long)Nx * Ny;
+  Kahan acc[2] = {};
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < N; i += (long)gridDim.x * 256) {
+    const int y = (int)(i / Nx), x = (int)(i - (long)y * Nx);
+    const size_t o = (size_t)(y + 1) * nx + (x + 1);
+    const double cr = (double)xCr[o], nb = (double)xNb[o];
+    const double fd = (double)pd[o], fl = (double)pl[o];
+    const double pD = p(fd), pL = p(fl), pG = 1.0 - pD - pL;
+    double mu0, mu1;
+    kks_mu(cr, nb, pD, pG, pL, mu0, mu1);
+    const double r0 = cr - pG * XE[0][0] - pD * XE[1][0] - pL * XE[2][0];
+    const double r1 = nb - pG * XE[0][1] - pD * XE[1][1] - pL * XE[2][1];
+    double g = 0.5 * (mu0 * r0 + mu1 * r1);
+    g += E.omega[0] * (fd * fd) * ((1.0 - fd) * (1.0 - fd));
+    g += E.omega[1] * (fl * fl) * ((1.0 - fl) * (1.0 - fl));
+    g += 2.0 * E.Omega * (fd * fd) * (fl * fl);
+    const size_t xl = (size_t)(y + 1) * nx + 1 + bc_map<BC>(x - 1, Nx);
+    const size_t xh = (size_t)(y + 1) * nx + 1 + bc_map<BC>(x + 1, Nx);
+    const size_t yl = (size_t)(1 + bc_map<BC>(y - 1, Ny)) * nx + (x + 1);
+    const size_t yh = (size_t)(1 + bc_map<BC>(y + 1, Ny)) * nx + (x + 1);
+    const T* phi[2] = {pd, pl};
+    double ge = 0.0;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const double gx = ((double)phi[k][xh] - (double)phi[k][xl]) / (2.0 * E.dx);
+      const double gy = ((double)phi[k][yh] - (double)phi[k][yl]) / (2.0 * E.dy);
+      ge += E.kappa[k] * (gx * gx + gy * gy);
+    }
+    const double c = g * E.dV, gg = ge * E.dV;
+    if (chem_out) chem_out[o] = c;
+    if (grad_out) grad_out[o] = gg;
+    kadd(acc[0], c);
+    kadd(acc[1], gg);
+  }
+  __shared__ Kahan sh[2][256];
+  for (int k = 0; k < 2; k++) sh[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 2; k++) kmerge(sh[k][threadIdx.x], sh[k][threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 2; k++) {
+      partials[(size_t)blockIdx.x * 4 + 2 * k] = sh[k][0].s;
+      partials[(size_t)blockIdx.x * 4 + 2 * k + 1] = sh[k][0].c;
+    }
+}
+
+__global__ void energy_total_kernel(const double* sums, double* out3) {
+  if (threadIdx.x == 0) {
+    out3[1] = sums[0];
+    out3[2] = sums[1];
+    out3[0] = sums[0] + sums[1];
+  }
+}
+
+template <typename T>
+hipError_t launch_energy(int nx, int ny, int bc, const EnergyParams& E, const T* xCr,
+                         const T* xNb, const T* pd, const T* pl, double* chem, double* grad,
+                         double* out3, const Scratch& S, hipStream_t st) {
+  const long N = (long)(nx - 2) * (ny - 2);
+  int blocks = (int)((N + 255) / 256);
+  if (blocks > REDUCE_BLOCKS) blocks = REDUCE_BLOCKS;
+  if (blocks < 1) blocks = 1;
+  if (bc == 0)
+    hipLaunchKernelGGL((energy_partial_kernel<T, 0>), dim3(blocks), dim3(256), 0, st, xCr, xNb,
+                       pd, pl, nx, ny, E, chem, grad, S.partials);
+  else
+    hipLaunchKernelGGL((energy_partial_kernel<T, 1>), dim3(blocks), dim3(256), 0, st, xCr, xNb,
+                       pd, pl, nx, ny, E, chem, grad, S.partials);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // the two sums land in out3[1..2] via a scratch slot after the partials
+  double* sums = S.partials + (size_t)REDUCE_BLOCKS * 10;
+  hipLaunchKernelGGL(reduce_final_kernel, dim3(2), dim3(64), 0, st, S.partials, blocks, 2, 1.0,
+                     sums);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(energy_total_kernel, dim3(1), dim3(64), 0, st, sums, out3);
+  return hipGetLastError();
+}
+template hipError_t launch_energy<double>(int, int, int, const EnergyParams&, const double*,
+                                          const double*, const double*, const double*, double*,
+                                          double*, double*, const Scratch&, hipStream_t);
+template hipError_t launch_energy<float>(int, int, int, const EnergyParams&, const float*,
+                                         const float*, const float*, const float*, double*,
+                                         double*, double*, const Scratch&, hipStream_t);
 
 // ------------------------------------------------------------ dataviz
 // src/discretization.cu:1211-1229 over every cell (ghosts included)

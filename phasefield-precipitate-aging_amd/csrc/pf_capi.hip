@@ -141,6 +141,30 @@ int summarize_t(pf_ctx* ctx, int nx, int ny, const T* xCr, const T* xNb, const T
   return PF_OK;
 }
 
+template <typename T>
+int energy_t(pf_ctx* ctx, int nx, int ny, int bc, double dx, double dy, const double* kappa,
+             const double* omega, double Omega, const T* xCr, const T* xNb, const T* pd,
+             const T* pl, double* chem, double* grad, double* out3, void* stream) {
+  if (!ctx || !kappa || !omega || !out3) return fail(PF_ERR_ARG, "null argument");
+  if (int r = check_grid(nx, ny)) return r;
+  if (int r = check_ptrs(xCr, xNb, pd, pl)) return r;
+  if (bc != PF_BC_NOFLUX && bc != PF_BC_PERIODIC) return fail(PF_ERR_ARG, "unknown boundary mode");
+  if (!(dx > 0.0) || !(dy > 0.0)) return fail(PF_ERR_ARG, "mesh spacing must be positive");
+  pf::EnergyParams E;
+  E.dx = dx;
+  E.dy = dy;
+  E.dV = dx * dy;
+  for (int k = 0; k < 2; k++) {
+    E.kappa[k] = kappa[k];
+    E.omega[k] = omega[k];
+  }
+  E.Omega = Omega;
+  hipError_t e = pf::launch_energy<T>(nx, ny, bc, E, xCr, xNb, pd, pl, chem, grad, out3,
+                                      ctx->scratch, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "energy kernels");
+  return PF_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -300,6 +324,22 @@ int pf_summarize_f64(pf_ctx* ctx, int nx, int ny, const double* a, const double*
 int pf_summarize_f32(pf_ctx* ctx, int nx, int ny, const float* a, const float* b,
                      const float* c, const float* d, double* out5, void* stream) {
   return summarize_t<float>(ctx, nx, ny, a, b, c, d, out5, stream);
+}
+
+int pf_energy_f64(pf_ctx* ctx, int nx, int ny, int bc, double dx, double dy,
+                  const double* kappa, const double* omega, double Omega, const double* a,
+                  const double* b, const double* c, const double* d, double* chem, double* grad,
+                  double* out3, void* stream) {
+  return energy_t<double>(ctx, nx, ny, bc, dx, dy, kappa, omega, Omega, a, b, c, d, chem, grad,
+                          out3, stream);
+}
+
+int pf_energy_f32(pf_ctx* ctx, int nx, int ny, int bc, double dx, double dy,
+                  const double* kappa, const double* omega, double Omega, const float* a,
+                  const float* b, const float* c, const float* d, double* chem, double* grad,
+                  double* out3, void* stream) {
+  return energy_t<float>(ctx, nx, ny, bc, dx, dy, kappa, omega, Omega, a, b, c, d, chem, grad,
+                         out3, stream);
 }
 
 int pf_dataviz_f64(int nx, int ny, const double* a, const double* b, const double* c,

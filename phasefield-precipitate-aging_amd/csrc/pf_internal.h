@@ -45,6 +45,15 @@ struct Event {
 };
 
 // Device scratch owned by a context: nucleation events and reduction partials.
+// Free-energy functional parameters (src/alloy625.cpp:865-919 with
+// src/parameters.h:36-48): mesh spacing, cell volume dx*dy, gradient and
+// well coefficients per phase, three-phase coefficient.
+struct EnergyParams {
+  double dx, dy, dV;
+  double kappa[2], omega[2];
+  double Omega;
+};
+
 struct Scratch {
   uint32_t* counters;  // [0] event count, [1] overflow flag, [2] wide-event total
   Event* events;
@@ -86,6 +95,10 @@ hipError_t launch_nucleation(int nx, int ny, T* xCr, T* xNb, T* pd, T* pl, const
 template <typename T>
 hipError_t launch_apply_events(int nx, int ny, T* xCr, T* xNb, T* pd, T* pl, const Scratch& S,
                                hipStream_t st);
+template <typename T>
+hipError_t launch_energy(int nx, int ny, int bc, const EnergyParams& E, const T* xCr,
+                         const T* xNb, const T* pd, const T* pl, double* chem, double* grad,
+                         double* out3, const Scratch& S, hipStream_t st);
 template <typename T>
 hipError_t launch_summarize(int nx, int ny, const T* xCr, const T* xNb, const T* pd,
                             const T* pl, double* out5, const Scratch& S, hipStream_t st);

@@ -70,6 +70,8 @@ PROTOTYPES = {
     "pf_summarize_f64": (_i, [_vp, _i, _i] + [_vp] * 5 + [_vp]),
     "pf_summarize_f32": (_i, [_vp, _i, _i] + [_vp] * 5 + [_vp]),
     "pf_dataviz_f64": (_i, [_i, _i] + [_vp] * 6 + [_vp]),
+    "pf_energy_f64": (_i, [_vp, _i, _i, _i, _d, _d, _dp, _dp, _d] + [_vp] * 7 + [_vp]),
+    "pf_energy_f32": (_i, [_vp, _i, _i, _i, _d, _d, _dp, _dp, _d] + [_vp] * 7 + [_vp]),
     "pf_thermo_eval": (_i, [_i, _i, _vp, _vp, _vp]),
     "pf_philox_eval": (_i, [_i, _vp, _vp, _vp]),
     "pf_default_rows": (_i, [_i, _i]),

@@ -9,6 +9,7 @@ Same names, argument order and meaning as the C++ functions the driver calls
     device_laplacian, device_laplacian_boundaries, device_evolution,
     device_nucleation, device_dataviz                   src/discretization.cuh:39-118
     swap_pointers_1D                                    src/mesh.h:44
+    summarize_fields, summarize_energy                  src/alloy625.cpp:811-919 (on the GPU)
 
 Device memory and streams come from torch (plumbing only); every computation
 runs in libpfpa.so.  Errors raise PfError (the reference asserts in checkCuda,
@@ -309,6 +310,38 @@ def summarize_fields(dev, nx, ny):
                                  _ptr(dev.phi_del_old), _ptr(dev.phi_lav_old), _ptr(out),
                                  _stream()), "summarize_fields")
     return out.cpu().tolist()
+
+
+def summarize_energy(dev, nx, ny, dx, dy, host=None, Omega=None):
+    """Total free energy of the latest fields, src/alloy625.cpp:865-919
+    (gibbs + kappa |grad phi|^2, times dV = dx dy, summed over the interior),
+    on the GPU with compensated sums.  kappa and omega are the values init_cuda
+    received; Omega defaults to src/parameters.h:37.  With `host`, the per-cell
+    energies go to host.chem_nrg / host.grad_nrg as the reference writes
+    them.  Returns (total, chemical, gradient)."""
+    from .model import OMEGA_3PHASE
+    device_flush(dev)
+    P = dev.params
+    kappa = (ctypes.c_double * 2)(P.kappa[0], P.kappa[1])
+    omega = (ctypes.c_double * 2)(P.omega[0], P.omega[1])
+    out = torch.zeros(3, dtype=torch.float64, device="cuda")
+    chem = grad = None
+    if host is not None:
+        chem = torch.zeros(ny * nx, dtype=torch.float64, device="cuda")
+        grad = torch.zeros(ny * nx, dtype=torch.float64, device="cuda")
+    check(lib().pf_energy_f64(dev.ctx, nx, ny, dev.bc, dx, dy, kappa, omega,
+                              OMEGA_3PHASE if Omega is None else Omega,
+                              _ptr(dev.conc_Cr_old), _ptr(dev.conc_Nb_old),
+                              _ptr(dev.phi_del_old), _ptr(dev.phi_lav_old),
+                              _ptr(chem) if chem is not None else None,
+                              _ptr(grad) if grad is not None else None, _ptr(out), _stream()),
+          "summarize_energy")
+    if host is not None:
+        c = chem.reshape(ny, nx).cpu().numpy()
+        g = grad.reshape(ny, nx).cpu().numpy()
+        host.chem_nrg[1:-1, 1:-1] = c[1:-1, 1:-1]
+        host.grad_nrg[1:-1, 1:-1] = g[1:-1, 1:-1]
+    return tuple(out.cpu().tolist())
 
 
 def nucleation_status(dev):

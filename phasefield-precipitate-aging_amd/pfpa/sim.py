@@ -139,6 +139,32 @@ class Simulation:
         self.stream.synchronize()
         return out.cpu().tolist()
 
+    def energy(self, maps=False):
+        """Free energy of the current fields (src/alloy625.cpp:865-919):
+        (total, chemical, gradient), and the per-cell chem / grad arrays when
+        `maps`."""
+        f64 = self.dtype == torch.float64
+        fn = lib().pf_energy_f64 if f64 else lib().pf_energy_f32
+        s = self.s
+        kappa = (ctypes.c_double * 2)(*s.kappa)
+        omega = (ctypes.c_double * 2)(*s.omega)
+        out = torch.zeros(3, dtype=torch.float64, device="cuda")
+        chem = grad = None
+        if maps:
+            chem = torch.zeros(self.nx * self.ny, dtype=torch.float64, device="cuda")
+            grad = torch.zeros_like(chem)
+        a = self.a
+        check(fn(self.ctx, self.nx, self.ny, s.bc, s.dx, s.dx, kappa, omega, s.alpha,
+                 _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(a[3]),
+                 _ptr(chem) if maps else None, _ptr(grad) if maps else None, _ptr(out),
+                 ctypes.c_void_p(self.stream.cuda_stream)), "pf_energy")
+        self.stream.synchronize()
+        tot = tuple(out.cpu().tolist())
+        if maps:
+            return tot, chem.reshape(self.ny, self.nx).cpu().numpy(), \
+                grad.reshape(self.ny, self.nx).cpu().numpy()
+        return tot
+
     def close(self):
         if self.ctx is not None and self.owns_ctx:
             torch.cuda.synchronize()

@@ -436,3 +436,89 @@ def test_nucleation_decisions_and_embed(oracle):
     assert ((probs > 0.05) & (probs < 0.95)).mean() > 0.5
     assert seen > 100
     assert flags[10:14, 10:14].sum() == 0 and flags[20:22, 5:9].sum() == 0
+
+
+# ------------------------------------------------------------ energy functional
+# orc_energy restates src/alloy625.cpp:752-919 (gibbs, maskedgradient,
+# summarize_energy).  Pinned by closed forms in which each term is isolated.
+
+E_DX = 2.5e-10
+E_KAPPA = (4.4318181818181824e-10, 4.4318181818181824e-10)
+E_OMEGA = (6.8637292464878e8, 6.8637292464878e8)
+E_OMEGA3 = 1.07e11
+
+
+def _uniform(Nx, Ny, xCr, xNb, pd, pl):
+    return [np.full((Ny + 2, Nx + 2), v, dtype=np.float64) for v in (xCr, xNb, pd, pl)]
+
+
+def test_energy_pure_phases_are_the_paraboloids(oracle):
+    """phi = 0: pGam = 1, x_gam = x, so the density is g_gam(x); phi_del = 1
+    (phi_lav = 0): pDel = 1, x_del = x and the well vanishes, so it is
+    g_del(x).  Ties the combination to the golden-pinned paraboloids."""
+    Nx, Ny = 7, 5
+    dV = E_DX * E_DX
+    for xCr, xNb in ((0.31, 0.021), (0.45, 0.11), (0.05, 0.2)):
+        f = _uniform(Nx, Ny, xCr, xNb, 0.0, 0.0)
+        tot, chem, grad = oracle.energy(f, 0, E_DX, E_DX, E_KAPPA, E_OMEGA, E_OMEGA3)
+        assert grad == 0.0
+        want = Nx * Ny * dV * oracle.g(0, xCr, xNb)
+        assert chem == pytest.approx(want, rel=1e-13)
+        f = _uniform(Nx, Ny, xCr, xNb, 1.0, 0.0)
+        tot, chem, grad = oracle.energy(f, 1, E_DX, E_DX, E_KAPPA, E_OMEGA, E_OMEGA3)
+        assert chem == pytest.approx(Nx * Ny * dV * oracle.g(1, xCr, xNb), rel=1e-12)
+        f = _uniform(Nx, Ny, xCr, xNb, 0.0, 1.0)
+        tot, chem, grad = oracle.energy(f, 1, E_DX, E_DX, E_KAPPA, E_OMEGA, E_OMEGA3)
+        assert chem == pytest.approx(Nx * Ny * dV * oracle.g(2, xCr, xNb), rel=1e-12)
+
+
+def test_energy_double_well_and_trijunction(oracle):
+    """At x = sum_a p_a xe_a the KKS potential is 0 and every g_a(x_a) = 0, so
+    only the Landau terms remain: omega/16 at phi = 1/2, 2 Omega at
+    phi_del = phi_lav = 1."""
+    Nx, Ny = 6, 4
+    dV = E_DX * E_DX
+    xe = lambda a: np.array([oracle.xe(a, 0), oracle.xe(a, 1)])  # noqa: E731
+    x = 0.5 * xe(0) + 0.5 * xe(1)
+    f = _uniform(Nx, Ny, x[0], x[1], 0.5, 0.0)
+    tot, chem, grad = oracle.energy(f, 1, E_DX, E_DX, E_KAPPA, E_OMEGA, E_OMEGA3)
+    assert chem == pytest.approx(Nx * Ny * dV * E_OMEGA[0] / 16, rel=1e-9)
+    x = xe(1) + xe(2) - xe(0)   # pGam = -1
+    f = _uniform(Nx, Ny, x[0], x[1], 1.0, 1.0)
+    tot, chem, grad = oracle.energy(f, 0, E_DX, E_DX, E_KAPPA, E_OMEGA, E_OMEGA3)
+    assert chem == pytest.approx(Nx * Ny * dV * 2 * E_OMEGA3, rel=1e-9)
+    assert tot == chem + grad
+
+
+def test_energy_gradient_closed_forms(oracle):
+    """Central differences: a linear ramp in phi_del with no-flux edges (a/dx
+    inside, a/(2 dx) at the two clamped columns) and a periodic sinusoid in
+    phi_lav along y (exact: A sin(theta)/dy cos(theta y))."""
+    Nx, Ny = 20, 3
+    a = 0.04
+    dx, dy = E_DX, 1.5 * E_DX
+    pd = np.zeros((Ny + 2, Nx + 2))
+    pd[:, 1:-1] = a * np.arange(1, Nx + 1)
+    pG = 1 - np.vectorize(oracle.p)(pd)
+    pD = np.vectorize(oracle.p)(pd)
+    xCr = pG * oracle.xe(0, 0) + pD * oracle.xe(1, 0)
+    xNb = pG * oracle.xe(0, 1) + pD * oracle.xe(1, 1)
+    f = [xCr, xNb, pd, np.zeros_like(pd)]
+    tot, chem, grad = oracle.energy(f, 0, dx, dy, E_KAPPA, E_OMEGA, E_OMEGA3)
+    dV = dx * dy
+    want_grad = E_KAPPA[0] * dV * Ny * ((Nx - 2) * (a / dx) ** 2 + 2 * (a / (2 * dx)) ** 2)
+    assert grad == pytest.approx(want_grad, rel=1e-12)
+    phi = pd[1:-1, 1:-1]
+    want_chem = dV * np.sum(E_OMEGA[0] * phi ** 2 * (1 - phi) ** 2)
+    assert chem == pytest.approx(want_chem, rel=1e-7, abs=1e-30)
+
+    Nx, Ny = 3, 16
+    A = 0.3
+    th = 2 * np.pi / Ny
+    pl = np.zeros((Ny + 2, Nx + 2))
+    yy = np.arange(Ny)[:, None] * np.ones((1, Nx))
+    pl[1:-1, 1:-1] = 0.5 + A * np.sin(th * yy)
+    f = [np.full_like(pl, 0.3), np.full_like(pl, 0.02), np.zeros_like(pl), pl]
+    _, _, grad = oracle.energy(f, 1, dx, dy, E_KAPPA, E_OMEGA, E_OMEGA3)
+    want = E_KAPPA[1] * dx * dy * Nx * (A * np.sin(th) / dy) ** 2 * Ny / 2
+    assert grad == pytest.approx(want, rel=1e-12)
