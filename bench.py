@@ -436,9 +436,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
-              "(multi-GPU sharding is described in DESIGN.md, not built)" % args.gpus,
-              file=sys.stderr)
+        print("bench.py: runs on one GPU; --gpus %d is not supported (the y-slab "
+              "sharding across GPUs is a design, not built)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     if args.calibrate_copy:
