@@ -135,9 +135,10 @@ def plain_section(args, dtype_name="f64"):
     cells = N * N
     value = cells * args.steps / (ms / 1e3)
 
-    # The dominant kernel alone: back-to-back launches of the two-step kernel
-    # (every step pair without a nucleation step), then of the one-step kernel
-    # (nucleation steps; the drop-in API's device_evolution), on the sim's stream.
+    # The kernels alone, back to back on the sim's stream: the three-step kernel
+    # (the dominant one), the two-step kernel (what the schedule leaves between
+    # nucleation steps) and the one-step kernel (nucleation steps; the drop-in
+    # API with grouping off).
     P = sim.P
     a, b = sim.a, sim.b
     st = ctypes.c_void_p(stream.cuda_stream)
@@ -415,6 +416,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-only", action="store_true")
+    ap.add_argument("--only", choices=["dropin", "batched", "serving"], default=None,
+                    help="run one of the other sections alone (no headline line)")
     ap.add_argument("--dropin-steps", type=int, default=900,
                     help="steps of the C++ drop-in replay (the driver's call sequence)")
     ap.add_argument("--batch-instances", type=int, default=512)
@@ -472,6 +475,14 @@ def main():
             sim.close()
         print(json.dumps({"kernels_only": [N, N], "launches": {"one_step": 1, "two_step": 10,
                                                                "three_step": 7}}))
+        return
+
+    if args.only:
+        # one section alone (for profiling), one JSON line
+        args.size = args.size or 4096
+        fn = {"dropin": dropin_section, "batched": batched_section,
+              "serving": serving_section}[args.only]
+        print(json.dumps({args.only: fn(args)}))
         return
 
     if args.form == "precision":

@@ -54,7 +54,14 @@ struct BatchHeader {
   int njobs;
   unsigned int done_blocks;      // last-block-done counter of the nucleation kernel
   unsigned int nuc_counters[4];  // [0] deferred events, [1] overflow flag, [2] wide events
+  int any_due;                   // some active job has a nucleation step in this launch
 };
+
+// the last step of this launch is one of the job's nucleation steps
+__host__ __device__ __forceinline__ bool nuc_due(const BatchJob& J) {
+  return J.active && J.nuc_every > 0 &&
+         (J.step + (uint64_t)J.adv - 1) % (uint64_t)J.nuc_every == 0;
+}
 
 constexpr int BATCH_RB = 16;  // reduction blocks per job
 
@@ -131,11 +138,14 @@ __global__ __launch_bounds__(WAVE * WPB) void batch_nuc_advance_kernel(
   int first, last, stride, w;
   xcd_range(hdr->total_waves, first, last, stride, w);
   const int lane = threadIdx.x & (WAVE - 1);
+  // launches without a due job (all but one in nuc_every / steps per launch)
+  // skip the scan of the wave table
+  if (!hdr->any_due) last = first;
   for (; w < last; w += stride) {
     const int j = __builtin_amdgcn_readfirstlane(wave_job[w]);
     const BatchJob& J = jobs[j];
     const uint64_t s_last = J.step + J.adv - 1;  // the step whose output this is
-    if (!J.active || J.nuc_every <= 0 || (s_last % (uint64_t)J.nuc_every) != 0) continue;
+    if (!nuc_due(J)) continue;
     const int local = w - J.wave0;
     const int strip = local % J.strips, chunk = local / J.strips;
     const int gx = strip * STRIP3 + lane - 1;
@@ -175,6 +185,7 @@ __global__ __launch_bounds__(WAVE * WPB) void batch_nuc_advance_kernel(
   __syncthreads();
   if (ticket == gridDim.x - 1) {
     __threadfence();
+    int due = 0;
     for (int j = threadIdx.x; j < hdr->njobs; j += blockDim.x) {
       BatchJob& J = jobs[j];
       if (J.active) {
@@ -182,10 +193,14 @@ __global__ __launch_bounds__(WAVE * WPB) void batch_nuc_advance_kernel(
         J.parity ^= 1;
         if (J.step >= J.target) J.active = 0;
         J.adv = next_adv(J);
+        due |= nuc_due(J) ? 1 : 0;
       }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) hdr->done_blocks = 0u;
+    due = __syncthreads_or(due);
+    if (threadIdx.x == 0) {
+      hdr->any_due = due;
+      hdr->done_blocks = 0u;
+    }
   }
 }
 
@@ -609,6 +624,9 @@ int pf_batch_commit(pf_batch* b) {
   h.total_waves = (int)wave_job.size();
   h.njobs = (int)b->jobs.size();
   h.done_blocks = 0u;
+  h.any_due = 0;
+  for (size_t j = 0; j < b->jobs.size(); j++)
+    if (b->slot_live[j] && pf::nuc_due(b->jobs[j])) h.any_due = 1;
   if ((e = hipMemcpy(b->d_hdr, &h, sizeof(h), hipMemcpyHostToDevice)) != hipSuccess)
     return bhip(e, "upload header");
   for (size_t j = 0; j < b->jobs.size(); j++) b->slot_on_device[j] = b->slot_live[j];
