@@ -16,6 +16,28 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def plan_launches(t0, n, max_steps, nuc_every=0):
+    """Launches that advance steps t0 .. t0+n-1: a list of (k, nucleation)
+    with k steps per launch.  A launch groups up to `max_steps` steps none of
+    which is a nucleation step (t % nuc_every == 0, src/main.cpp:307-316);
+    a nucleation step runs alone with the nucleation pass (nucleation True).
+    nuc_every 0 = nucleation off."""
+    def due(t):
+        return nuc_every > 0 and t % nuc_every == 0
+    out = []
+    t, left = t0, n
+    while left > 0:
+        k = 1
+        for cand in (3, 2):
+            if cand <= max_steps and left >= cand and not any(due(t + j) for j in range(cand)):
+                k = cand
+                break
+        out.append((k, k == 1 and due(t)))
+        t += k
+        left -= k
+    return out
+
+
 class Simulation:
     """Double-buffered fields resident on the GPU.
 
@@ -91,21 +113,13 @@ class Simulation:
     def set_tile3(self, rows, depth=0):
         check(lib().pf_ctx_set_tile3(self.ctx, rows, depth), "pf_ctx_set_tile3")
 
-    def _nuc_due(self, t):
-        return self.nucleation and t % self.nuc_every == 0
-
     def step(self, n=1):
         st = ctypes.c_void_p(self.stream.cuda_stream)
-        left = n
-        while left > 0:
+        plan = plan_launches(self.steps_done, n, self.steps_per_launch,
+                             self.nuc_every if self.nucleation else 0)
+        for k, nucleate in plan:
             a, b = self.a, self.b
             t = self.steps_done
-            k = 1
-            for cand in (3, 2):
-                if (cand <= self.steps_per_launch and left >= cand
-                        and not any(self._nuc_due(t + j) for j in range(cand))):
-                    k = cand
-                    break
             if k > 1:
                 check(self._stepk[k](self.ctx, self.nx, self.ny, self.s.bc, ctypes.byref(self.P),
                                      _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(a[3]),
@@ -114,7 +128,7 @@ class Simulation:
                 done = k
             else:
                 nuc = None
-                if self._nuc_due(t):
+                if nucleate:
                     self.N.call = t // self.nuc_every
                     nuc = ctypes.byref(self.N)
                 check(self._step(self.ctx, self.nx, self.ny, self.s.bc, ctypes.byref(self.P),
@@ -124,7 +138,6 @@ class Simulation:
                 done = 1
             self.a, self.b = self.b, self.a
             self.steps_done += done
-            left -= done
 
     def fields(self):
         self.stream.synchronize()

@@ -1,0 +1,85 @@
+"""Host-side logic of the package (no GPU): the launch plan of the stepping
+loop, the batched and serving workloads."""
+import math
+
+import numpy as np
+import pytest
+
+from pfpa import model as M
+from pfpa.serve import make_requests
+from pfpa.sim import plan_launches
+from pfpa.workloads import batch_specs
+
+
+@pytest.mark.parametrize("max_steps", [1, 2, 3])
+@pytest.mark.parametrize("nuc_every", [0, 1, 2, 3, 4, 7, 891])
+@pytest.mark.parametrize("t0", [0, 1, 5, 890])
+def test_plan_launches(t0, max_steps, nuc_every):
+    """Every step exactly once, in order; groups never hold a nucleation step;
+    every nucleation step runs alone with the pass; groups are as long as the
+    rules allow (a shorter group is followed by a nucleation step or the end)."""
+    for n in (0, 1, 2, 3, 5, 17, 100):
+        plan = plan_launches(t0, n, max_steps, nuc_every)
+        assert sum(k for k, _ in plan) == n
+        t = t0
+        for i, (k, nuc) in enumerate(plan):
+            due = [nuc_every > 0 and (t + j) % nuc_every == 0 for j in range(k)]
+            assert 1 <= k <= max_steps
+            if k > 1:
+                assert not any(due) and not nuc
+            else:
+                assert nuc == due[0]
+            if k < max_steps and i + 1 < len(plan):
+                # cut short only by a nucleation step within reach
+                reach = min(max_steps, n - (t - t0))
+                assert any(nuc_every > 0 and (t + j) % nuc_every == 0 for j in range(reach))
+            t += k
+
+
+def test_plan_launches_schedule_of_the_benchmark():
+    """Nucleation every 891 steps from step 0: 1000 steps are one nucleation
+    launch, three-step launches and the groups the schedule leaves."""
+    plan = plan_launches(0, 1000, 3, 891)
+    assert plan[0] == (1, True)
+    nuc_steps = []
+    t = 0
+    for k, nuc in plan:
+        if nuc:
+            nuc_steps.append(t)
+        t += k
+    assert nuc_steps == [0, 891]
+    assert sum(1 for k, _ in plan if k == 3) >= 330
+
+
+def test_batch_specs():
+    specs = batch_specs(512, 128, 1024, seed=7)
+    assert len(specs) == 512
+    assert specs == batch_specs(512, 128, 1024, seed=7)      # seeded
+    n = np.array([sp["n"] for sp in specs])
+    assert n.min() >= 128 and n.max() <= 1024
+    # log-uniform: median near the geometric mean sqrt(128 * 1024) = 362
+    assert 300 < np.median(n) < 440
+    cells = (n.astype(float) ** 2).sum()
+    assert 1.1e8 < cells < 1.6e8                                # E = 2.48e5 per instance
+    for sp in specs:
+        assert 0 <= sp["n_del"] + sp["n_lav"] <= 8
+        cr, nb = sp["x0"]
+        enriched = 0.26 <= cr <= 0.29 and 0.15 <= nb <= 0.18
+        matrix = (M.MATRIX_CR[0] <= cr <= M.MATRIX_CR[1] and
+                  M.MATRIX_NB[0] <= nb <= M.MATRIX_NB[1])
+        assert enriched or matrix
+
+
+def test_requests_poisson():
+    rate, window = 6.0, 200.0
+    reqs = make_requests(rate, window, seed=3)
+    assert reqs == sorted(reqs, key=lambda r: r.arrival)
+    arr = np.array([r.arrival for r in reqs])
+    assert arr.min() >= 0 and arr.max() <= window
+    # count ~ Poisson(rate * window): within 5 sigma
+    assert abs(len(reqs) - rate * window) < 5 * math.sqrt(rate * window)
+    gaps = np.diff(arr)
+    assert abs(gaps.mean() - 1 / rate) < 0.1 / rate
+    for r in reqs:
+        assert 128 <= r.n <= 2048 and 500 <= r.steps <= 20000
+    assert len({r.seed for r in reqs}) == len(reqs)
