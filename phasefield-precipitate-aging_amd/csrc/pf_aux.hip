@@ -63,13 +63,25 @@ StepParams make_step(double dt, const double* DCr, const double* DNb, const doub
                      const double* omega, const double* Lmob, double alpha,
                      const double* mask) {
   StepParams P{};
-  P.cCr0 = dt * DCr[0];
-  P.cCr1 = dt * DCr[1];
-  P.cNb0 = dt * DNb[0];
-  P.cNb1 = dt * DNb[1];
+  // q = dt D (xe_g + A_g mu): coefficients of mu0, mu1 and the constant
+  const double* rows[2] = {DCr, DNb};
+  double* q[2] = {P.qCr, P.qNb};
+  for (int i = 0; i < 2; i++) {
+    q[i][0] = dt * (rows[i][0] * A[0][0] + rows[i][1] * A[0][1]);
+    q[i][1] = dt * (rows[i][0] * A[0][1] + rows[i][1] * A[0][2]);
+    q[i][2] = dt * (rows[i][0] * XE[0][0] + rows[i][1] * XE[0][1]);
+  }
+  // 30 dt L (mu.B.mu - mu.e) as a quadratic form in mu0, mu1
+  const double* B[2] = {B_DEL, B_LAV};
+  const double* E[2] = {E_DEL, E_LAV};
   for (int i = 0; i < 2; i++) {
     const double dtL = dt * Lmob[i];
-    P.a[i] = 30.0 * dtL;
+    const double a = 30.0 * dtL;
+    P.g[i][0] = a * B[i][0];
+    P.g[i][1] = a * (2.0 * B[i][1]);
+    P.g[i][2] = a * B[i][2];
+    P.g[i][3] = -a * E[i][0];
+    P.g[i][4] = -a * E[i][1];
     P.w[i] = 2.0 * omega[i] * dtL;
     P.c[i] = 2.0 * alpha * dtL;
     P.k[i] = kappa[i] * dtL;

@@ -10,19 +10,23 @@
 namespace pf {
 
 // Kinetic and discretisation parameters of one step (by value into kernels),
-// with dt and the mobilities folded in on the host:
-//   x_Cr' = x_Cr + cCr0 lap(gCr) + cCr1 lap(gNb)            (dt * D_Cr row)
-//   phi'  = phi + dtL [30 u^2 dOmega + 2 omega u (2 phi - 1)
-//                      - 2 alpha phi phi_o^2 + kappa lap(phi)],  u = phi (1 - phi)
-// which is src/discretization.cu:859-863 and 888-942 with
-// p'(phi) = 30 u^2 and phi (phi - 1)(2 phi - 1) = -u (2 phi - 1).
+// with dt, the mobilities and the constant prefactors folded in on the host.
+// The gamma fictitious composition x_g = xe_g + A_g mu is linear in the
+// potential mu, and so is the Laplacian, so the Cahn-Hilliard update
+// x' = x + dt D lap(x_g) (src/discretization.cu:859-863) is
+//   x_Cr' = x_Cr + lap(q_Cr),  q_Cr = qCr[0] mu0 + qCr[1] mu1 + qCr[2]
+// with q_Cr = dt (D_CrCr x_g,Cr + D_CrNb x_g,Nb), and likewise for Nb.  The
+// Allen-Cahn update (src/discretization.cu:888-942) is
+//   phi' = phi + u (G(mu) u + w (2 phi - 1)) - c phi phi_o^2 + k lap(phi)
+// with u = phi (1 - phi), p'(phi) = 30 u^2, phi (phi - 1)(2 phi - 1) = -u (2 phi - 1)
+// and G = 30 dt L dOmega = g[0] mu0^2 + g[1] mu0 mu1 + g[2] mu1^2 + g[3] mu0 + g[4] mu1.
 struct StepParams {
-  double cCr0, cCr1, cNb0, cNb1;  // dt * D rows
-  double a[2];                    // 30 dt L        (delta, Laves)
-  double w[2];                    // 2 omega dt L
-  double c[2];                    // 2 alpha dt L
-  double k[2];                    // kappa dt L
-  double m[9];                    // 3x3 Laplacian mask, row-major (y-1, y, y+1)
+  double qCr[3], qNb[3];  // dt * (D row) . (A_gamma mu + xe_gamma)
+  double g[2][5];         // 30 dt L * grand-potential difference  (delta, Laves)
+  double w[2];            // 2 omega dt L
+  double c[2];            // 2 alpha dt L
+  double k[2];            // kappa dt L
+  double m[9];            // 3x3 Laplacian mask, row-major (y-1, y, y+1)
 };
 
 // Nucleation pass parameters, derived on the host from the reference's

@@ -6,7 +6,7 @@
 // One wave64 owns a strip of 64 columns (62 output columns + one halo column
 // each side) and marches down a chunk of rows.  Each lane keeps a three-row
 // window (y-1, y, y+1) of what the stencil needs -- phi_delta, phi_Laves and
-// the fictitious gamma composition -- in registers, so each cell is loaded
+// the composition potentials q = dt D x_gamma -- in registers, so each cell is loaded
 // once per strip and its KKS solve is done once.  x-neighbours come from
 // adjacent lanes through DPP wave shifts (no LDS round trip).  Loads run DEPTH
 // rows ahead of the row being computed (a register ring), so a wave keeps
@@ -67,7 +67,7 @@ __device__ __forceinline__ double from_right(double v) {
 
 struct Point {       // what the neighbours of a cell need
   double pd, pl;     // phase fields
-  double gCr, gNb;   // fictitious gamma composition
+  double qCr, qNb;   // dt D x_gamma: the Cahn-Hilliard update is x' = x + lap(q)
 };
 struct Centre {      // extra state of a cell when it is the stencil centre
   double xCr, xNb;   // compositions
@@ -117,28 +117,36 @@ __device__ __forceinline__ double recip(double d) {
   return fma(r, e, r);
 }
 
-// interpolator p(x) = x^3 (6x^2 - 15x + 10) in Horner form
-__device__ __forceinline__ double p_interp(double x) {
-  return (x * x) * x * fma(x, fma(6.0, x, -15.0), 10.0);
+// interpolator p(x) = x^3 (6x^2 - 15x + 10), returned divided by 6:
+// x^3 (x (x - 5/2) + 5/3).  The factor 6 is folded into the constants p
+// multiplies (DA_*, DXE_*), which keeps every operation to one constant
+// operand.
+__device__ __forceinline__ double p_interp6(double x) {
+  return (x * x) * x * fma(x, x - 2.5, 5.0 / 3.0);
 }
 
 // KKS solve of a cell (src/parabola625.cu:245-297 restated): with
 // p_gamma = 1 - p_delta - p_Laves eliminated, M = A_g + p_d (A_d - A_g) +
 // p_l (A_l - A_g) and r = x - xe_g - p_d (xe_d - xe_g) - p_l (xe_l - xe_g);
-// mu = M^-1 r and the gamma fictitious composition x_g = xe_g + A_g mu.
-constexpr double DA_DEL[3] = {A[1][0] - A[0][0], A[1][1] - A[0][1], A[1][2] - A[0][2]};
-constexpr double DA_LAV[3] = {A[2][0] - A[0][0], A[2][1] - A[0][1], A[2][2] - A[0][2]};
-constexpr double DXE_DEL[2] = {XE[1][0] - XE[0][0], XE[1][1] - XE[0][1]};
-constexpr double DXE_LAV[2] = {XE[2][0] - XE[0][0], XE[2][1] - XE[0][1]};
+// mu = M^-1 r, and the neighbours need q = dt D x_g, x_g = xe_g + A_g mu
+// (linear in mu; coefficients in StepParams).  The constants here carry the
+// factor 6 of p_interp6.
+constexpr double DA_DEL[3] = {6.0 * (A[1][0] - A[0][0]), 6.0 * (A[1][1] - A[0][1]),
+                              6.0 * (A[1][2] - A[0][2])};
+constexpr double DA_LAV[3] = {6.0 * (A[2][0] - A[0][0]), 6.0 * (A[2][1] - A[0][1]),
+                              6.0 * (A[2][2] - A[0][2])};
+constexpr double DXE_DEL[2] = {6.0 * (XE[1][0] - XE[0][0]), 6.0 * (XE[1][1] - XE[0][1])};
+constexpr double DXE_LAV[2] = {6.0 * (XE[2][0] - XE[0][0]), 6.0 * (XE[2][1] - XE[0][1])};
 
 template <typename T>
-__device__ __forceinline__ void derive(const Raw<T>& r, Point& v, Centre& c) {
+__device__ __forceinline__ void derive(const StepParams& P, const Raw<T>& r, Point& v,
+                                       Centre& c) {
   v.pd = (double)r.pd;
   v.pl = (double)r.pl;
   c.xCr = (double)r.cr;
   c.xNb = (double)r.nb;
-  const double pD = p_interp(v.pd);
-  const double pL = p_interp(v.pl);
+  const double pD = p_interp6(v.pd);
+  const double pL = p_interp6(v.pl);
   const double M0 = fma(pD, DA_DEL[0], fma(pL, DA_LAV[0], A[0][0]));
   const double M1 = fma(pD, DA_DEL[1], fma(pL, DA_LAV[1], A[0][1]));
   const double M2 = fma(pD, DA_DEL[2], fma(pL, DA_LAV[2], A[0][2]));
@@ -147,15 +155,15 @@ __device__ __forceinline__ void derive(const Raw<T>& r, Point& v, Centre& c) {
   const double inv = recip(fma(M0, M2, -(M1 * M1)));
   c.mu0 = fma(M2, r0, -(M1 * r1)) * inv;
   c.mu1 = fma(M0, r1, -(M1 * r0)) * inv;
-  v.gCr = fma(A[0][0], c.mu0, fma(A[0][1], c.mu1, XE[0][0]));
-  v.gNb = fma(A[0][1], c.mu0, fma(A[0][2], c.mu1, XE[0][1]));
+  v.qCr = fma(P.qCr[0], c.mu0, fma(P.qCr[1], c.mu1, P.qCr[2]));
+  v.qNb = fma(P.qNb[0], c.mu0, fma(P.qNb[1], c.mu1, P.qNb[2]));
 }
 
 __device__ __forceinline__ Point shift_left(const Point& a) {
-  return Point{from_left(a.pd), from_left(a.pl), from_left(a.gCr), from_left(a.gNb)};
+  return Point{from_left(a.pd), from_left(a.pl), from_left(a.qCr), from_left(a.qNb)};
 }
 __device__ __forceinline__ Point shift_right(const Point& a) {
-  return Point{from_right(a.pd), from_right(a.pl), from_right(a.gCr), from_right(a.gNb)};
+  return Point{from_right(a.pd), from_right(a.pl), from_right(a.qCr), from_right(a.qNb)};
 }
 
 // Nucleation of one cell from its new values (centre-only scan; the caller
@@ -203,27 +211,30 @@ __device__ __forceinline__ void nucleate_cell(const NucParams& N, uint32_t cell,
   }
 }
 
-// Laplacians of the four stencil quantities in the reference's mask order
+// Mask sums of the four stencil quantities in the reference's mask order
 // (src/discretization.cu:589-595): rows y-1, y, y+1; columns x-1, x, x+1.
+// The composition sums start from the centre's composition, so they are the
+// new compositions x + lap(q); the phase-field sums are the Laplacians.
 template <bool NINE>
 __device__ __forceinline__ void laplacians(const double* m, const Point& S, const Point& W,
                                            const Point& C, const Point& E, const Point& Nn,
                                            const Point& SW, const Point& SE, const Point& NW,
-                                           const Point& NE, double lap[4]) {
-  const double s[4] = {S.pd, S.pl, S.gCr, S.gNb};
-  const double w[4] = {W.pd, W.pl, W.gCr, W.gNb};
-  const double c[4] = {C.pd, C.pl, C.gCr, C.gNb};
-  const double e[4] = {E.pd, E.pl, E.gCr, E.gNb};
-  const double n[4] = {Nn.pd, Nn.pl, Nn.gCr, Nn.gNb};
+                                           const Point& NE, const Centre& cc, double lap[4]) {
+  const double s[4] = {S.pd, S.pl, S.qCr, S.qNb};
+  const double w[4] = {W.pd, W.pl, W.qCr, W.qNb};
+  const double c[4] = {C.pd, C.pl, C.qCr, C.qNb};
+  const double e[4] = {E.pd, E.pl, E.qCr, E.qNb};
+  const double n[4] = {Nn.pd, Nn.pl, Nn.qCr, Nn.qNb};
+  const double base[4] = {0.0, 0.0, cc.xCr, cc.xNb};
 #pragma unroll
   for (int f = 0; f < 4; f++) {
     double v;
     if (NINE) {
-      const double sw[4] = {SW.pd, SW.pl, SW.gCr, SW.gNb};
-      const double se[4] = {SE.pd, SE.pl, SE.gCr, SE.gNb};
-      const double nw[4] = {NW.pd, NW.pl, NW.gCr, NW.gNb};
-      const double ne[4] = {NE.pd, NE.pl, NE.gCr, NE.gNb};
-      v = m[0] * sw[f];
+      const double sw[4] = {SW.pd, SW.pl, SW.qCr, SW.qNb};
+      const double se[4] = {SE.pd, SE.pl, SE.qCr, SE.qNb};
+      const double nw[4] = {NW.pd, NW.pl, NW.qCr, NW.qNb};
+      const double ne[4] = {NE.pd, NE.pl, NE.qCr, NE.qNb};
+      v = f < 2 ? m[0] * sw[f] : fma(m[0], sw[f], base[f]);
       v = fma(m[1], s[f], v);
       v = fma(m[2], se[f], v);
       v = fma(m[3], w[f], v);
@@ -233,7 +244,7 @@ __device__ __forceinline__ void laplacians(const double* m, const Point& S, cons
       v = fma(m[7], n[f], v);
       v = fma(m[8], ne[f], v);
     } else {
-      v = m[1] * s[f];
+      v = f < 2 ? m[1] * s[f] : fma(m[1], s[f], base[f]);
       v = fma(m[3], w[f], v);
       v = fma(m[4], c[f], v);
       v = fma(m[5], e[f], v);
@@ -243,29 +254,24 @@ __device__ __forceinline__ void laplacians(const double* m, const Point& S, cons
   }
 }
 
-// grand-potential difference gamma - a at potential mu, Horner form of
-// mu.B_a.mu - mu.e_a (B_a = (A_a - A_gamma)/2, e_a = xe_gamma - xe_a)
-__device__ __forceinline__ double grand_potential_diff(const double (&B)[3], const double (&E)[2],
-                                                       double mu0, double mu1) {
-  return fma(mu1, fma(B[2], mu1, -E[1]), mu0 * fma(B[0], mu0, fma(2.0 * B[1], mu1, -E[0])));
-}
-
-// The explicit Euler update of one cell from its stencil Laplacians:
+// The explicit Euler update of one cell from its mask sums (laplacians()):
 // Cahn-Hilliard (src/discretization.cu:859-863) and Allen-Cahn
 // (src/discretization.cu:888-942) with the closed-form grand-potential
-// difference at the solved potential; the coefficients carry dt, L and the
-// constant prefactors (StepParams).
+// difference at the solved potential, as the quadratic form of StepParams.
 __device__ __forceinline__ void update_cell(const StepParams& P, const double lap[4],
                                             const Point& cur, const Centre& cc, double& nCr,
                                             double& nNb, double& npd, double& npl) {
-  nCr = fma(P.cCr0, lap[2], fma(P.cCr1, lap[3], cc.xCr));
-  nNb = fma(P.cNb0, lap[2], fma(P.cNb1, lap[3], cc.xNb));
+  nCr = lap[2];
+  nNb = lap[3];
   const double fd = cur.pd, fl = cur.pl;
-  const double dOd = grand_potential_diff(B_DEL, E_DEL, cc.mu0, cc.mu1);
-  const double dOl = grand_potential_diff(B_LAV, E_LAV, cc.mu0, cc.mu1);
+  const double m00 = cc.mu0 * cc.mu0, m01 = cc.mu0 * cc.mu1, m11 = cc.mu1 * cc.mu1;
+  const double Gd = fma(P.g[0][0], m00, fma(P.g[0][1], m01, fma(P.g[0][2], m11,
+                    fma(P.g[0][3], cc.mu0, P.g[0][4] * cc.mu1))));
+  const double Gl = fma(P.g[1][0], m00, fma(P.g[1][1], m01, fma(P.g[1][2], m11,
+                    fma(P.g[1][3], cc.mu0, P.g[1][4] * cc.mu1))));
   const double ud = fma(-fd, fd, fd), ul = fma(-fl, fl, fl);  // phi (1 - phi)
-  const double td = ud * fma(P.a[0] * ud, dOd, P.w[0] * fma(2.0, fd, -1.0));
-  const double tl = ul * fma(P.a[1] * ul, dOl, P.w[1] * fma(2.0, fl, -1.0));
+  const double td = ud * fma(ud, Gd, P.w[0] * fma(2.0, fd, -1.0));
+  const double tl = ul * fma(ul, Gl, P.w[1] * fma(2.0, fl, -1.0));
   npd = fma(-P.c[0] * fd, fl * fl, fma(P.k[0], lap[0], fd) + td);
   npl = fma(-P.c[1] * fl, fd * fd, fma(P.k[1], lap[1], fl) + tl);
 }
@@ -348,9 +354,9 @@ __device__ __forceinline__ void march(
   {
     Raw<T> r;
     load(r, y0 - 1);
-    derive<T>(r, win[2], cen[2]);
+    derive<T>(P, r, win[2], cen[2]);
     load(r, y0);
-    derive<T>(r, win[0], cen[0]);
+    derive<T>(P, r, win[0], cen[0]);
   }
 #pragma unroll
   for (int j = 0; j < DEPTH; j++) load(ring[j], min(y0 + 1 + j, y1));
@@ -369,7 +375,7 @@ __device__ __forceinline__ void march(
       if (y >= y1) return;
       const int ps = (k + 2) % 3, cs = k % 3, ns = (k + 1) % 3;
       const int rs = k % DEPTH;
-      derive<T>(ring[rs], win[ns], cen[ns]);
+      derive<T>(P, ring[rs], win[ns], cen[ns]);
       load(ring[rs], min(y + 1 + DEPTH, y1));
 
       Point W, E;
@@ -384,7 +390,7 @@ __device__ __forceinline__ void march(
       }
       double lap[4];
       laplacians<NINE>(m, win[ps], W, win[cs], E, win[ns], winW[ps], winE[ps], winW[ns],
-                       winE[ns], lap);
+                       winE[ns], cen[cs], lap);
 
       double nCr, nNb, npd, npl;
       update_cell(P, lap, win[cs], cen[cs], nCr, nNb, npd, npl);
@@ -483,7 +489,7 @@ __device__ __forceinline__ void marchk(
                       Point& v, Centre& c) {
     const Point W = shift_left(C), E = shift_right(C);
     double lap[4];
-    laplacians<false>(m, Sth, W, C, E, Nth, Sth, Sth, Nth, Nth, lap);
+    laplacians<false>(m, Sth, W, C, E, Nth, Sth, Sth, Nth, Nth, cc, lap);
     double n0, n1, n2, n3;
     update_cell(P, lap, C, cc, n0, n1, n2, n3);
     Raw<T> r;
@@ -503,7 +509,7 @@ __device__ __forceinline__ void marchk(
       r.pd = (T)(lo ? fr2 : (hi ? fl2 : c2));
       r.pl = (T)(lo ? fr3 : (hi ? fl3 : c3));
     }
-    derive<T>(r, v, c);
+    derive<T>(P, r, v, c);
   };
 
   // Levels 1..K-1 at iteration t (slots are compile-time in both callers).
@@ -538,7 +544,7 @@ __device__ __forceinline__ void marchk(
       const int sa = t % 3, sc = (t + 2) % 3, sb = (t + 1) % 3;
       Raw<T> r;
       load(r, y0 - K + t);
-      derive<T>(r, win[0][sa], cen[0][sa]);
+      derive<T>(P, r, win[0][sa], cen[0][sa]);
       if (t == 2) {  // start the ring once the first window is full
 #pragma unroll
         for (int j = 0; j < DEPTH; j++) load(ring[j], min(y0 + K + j, y_last));
@@ -557,7 +563,7 @@ __device__ __forceinline__ void marchk(
       const int t = 2 * K + ib + k;
       const int sa = (2 * K + k) % 3, sc = (2 * K + k + 2) % 3, sb = (2 * K + k + 1) % 3;
       const int rs = k % DEPTH;
-      derive<T>(ring[rs], win[0][sa], cen[0][sa]);  // old row y + K
+      derive<T>(P, ring[rs], win[0][sa], cen[0][sa]);  // old row y + K
       load(ring[rs], min(y + K + DEPTH, y_last));
       middle_levels(t, sa, sc, sb, K - 1, false);
       // level K: row y from level K-1's rows y-1 (behind), y, y+1 (ahead)
@@ -566,7 +572,7 @@ __device__ __forceinline__ void marchk(
       const Point& Nth = win[K - 1][sa];
       const Point W = shift_left(C), E = shift_right(C);
       double lap[4];
-      laplacians<false>(m, Sth, W, C, E, Nth, Sth, Sth, Nth, Nth, lap);
+      laplacians<false>(m, Sth, W, C, E, Nth, Sth, Sth, Nth, Nth, cen[K - 1][sc], lap);
       double nCr, nNb, npd, npl;
       update_cell(P, lap, C, cen[K - 1][sc], nCr, nNb, npd, npl);
       if (out_lane) {
