@@ -410,7 +410,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=20)
+    # the GPU's clock takes about 30 ms of sustained load to settle (the first
+    # launches of a cold run are up to 30% slower): 600 steps are ~55 ms
+    ap.add_argument("--warmup", type=int, default=600)
     ap.add_argument("--form", choices=["plain", "precision"], default="plain")
     ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=100)

@@ -397,6 +397,46 @@ def test_nine_point_mask(gpu, oracle, bc):
         assert mixed_err(got[k], want[k]) <= TOL, k
 
 
+def _masks(dx, dy):
+    """The reference's five- and nine-point masks (src/mesh.cpp:187-213)."""
+    five = [0.0, 1 / (dy * dy), 0.0,
+            1 / (dx * dx), -2 * (dx * dx + dy * dy) / (dx * dx * dy * dy), 1 / (dx * dx),
+            0.0, 1 / (dy * dy), 0.0]
+    c = 1 / (6 * dx * dy)
+    nine = [c, 4 / (6 * dy * dy), c,
+            4 / (6 * dx * dx), -10 * (dx * dx + dy * dy) / (6 * dx * dx * dy * dy),
+            4 / (6 * dx * dx),
+            c, 4 / (6 * dy * dy), c]
+    return five, nine
+
+
+@pytest.mark.parametrize("nine", [False, True])
+@pytest.mark.parametrize("bc", [0, 1])
+def test_anisotropic_masks(gpu, oracle, bc, nine):
+    """Masks for dy = 1.3 dx.  The reference's nine-point mask then does not
+    sum to zero (its entries sum to -2 (dx - dy)^2 / (6 dx^2 dy^2)), so the
+    mask sum of a uniform field is not zero and the constant part of
+    q = dt D x_gamma contributes to every update; the five-point mask sums to
+    zero.  One step per launch and, for the five-point mask, three."""
+    Nx, Ny = 96, 70
+    dx = 0.25e-9
+    dy = 1.3 * dx
+    s = M.make_setup(Nx, Ny, bc=bc, dt_rule=5e-8)
+    five, ninem = _masks(dx, dy)
+    s.mask = ninem if nine else five
+    if nine:
+        assert abs(sum(ninem)) > 1e-3 * max(abs(v) for v in ninem)
+    f0 = M.init_matrix_with_particles(Nx, Ny, dx, bc, seed=8, radius=2e-9)
+    want = oracle.run(f0, oracle_params(oracle, s), bc, 25)
+    for spl in ((1,) if nine else (1, 3)):
+        sim = Simulation(s, f0, steps_per_launch=spl)
+        sim.step(25)
+        got = sim.fields()
+        sim.close()
+        for k in range(4):
+            assert mixed_err(got[k], want[k]) <= TOL, (spl, k)
+
+
 def test_wide_nucleation_events(gpu, oracle):
     """Scan radius over the whole grid and embeds of radius ~1.7 cells that
     overlap: the deferred-event path, in the oracle's order."""
