@@ -211,21 +211,18 @@ __device__ __forceinline__ void nucleate_cell(const NucParams& N, uint32_t cell,
   }
 }
 
-// Mask sums of the four stencil quantities in the reference's mask order
+// Laplacians of the four stencil quantities in the reference's mask order
 // (src/discretization.cu:589-595): rows y-1, y, y+1; columns x-1, x, x+1.
-// The composition sums start from the centre's composition, so they are the
-// new compositions x + lap(q); the phase-field sums are the Laplacians.
 template <bool NINE>
 __device__ __forceinline__ void laplacians(const double* m, const Point& S, const Point& W,
                                            const Point& C, const Point& E, const Point& Nn,
                                            const Point& SW, const Point& SE, const Point& NW,
-                                           const Point& NE, const Centre& cc, double lap[4]) {
+                                           const Point& NE, double lap[4]) {
   const double s[4] = {S.pd, S.pl, S.qCr, S.qNb};
   const double w[4] = {W.pd, W.pl, W.qCr, W.qNb};
   const double c[4] = {C.pd, C.pl, C.qCr, C.qNb};
   const double e[4] = {E.pd, E.pl, E.qCr, E.qNb};
   const double n[4] = {Nn.pd, Nn.pl, Nn.qCr, Nn.qNb};
-  const double base[4] = {0.0, 0.0, cc.xCr, cc.xNb};
 #pragma unroll
   for (int f = 0; f < 4; f++) {
     double v;
@@ -234,7 +231,7 @@ __device__ __forceinline__ void laplacians(const double* m, const Point& S, cons
       const double se[4] = {SE.pd, SE.pl, SE.qCr, SE.qNb};
       const double nw[4] = {NW.pd, NW.pl, NW.qCr, NW.qNb};
       const double ne[4] = {NE.pd, NE.pl, NE.qCr, NE.qNb};
-      v = f < 2 ? m[0] * sw[f] : fma(m[0], sw[f], base[f]);
+      v = m[0] * sw[f];
       v = fma(m[1], s[f], v);
       v = fma(m[2], se[f], v);
       v = fma(m[3], w[f], v);
@@ -244,7 +241,7 @@ __device__ __forceinline__ void laplacians(const double* m, const Point& S, cons
       v = fma(m[7], n[f], v);
       v = fma(m[8], ne[f], v);
     } else {
-      v = f < 2 ? m[1] * s[f] : fma(m[1], s[f], base[f]);
+      v = m[1] * s[f];
       v = fma(m[3], w[f], v);
       v = fma(m[4], c[f], v);
       v = fma(m[5], e[f], v);
@@ -254,15 +251,18 @@ __device__ __forceinline__ void laplacians(const double* m, const Point& S, cons
   }
 }
 
-// The explicit Euler update of one cell from its mask sums (laplacians()):
+// The explicit Euler update of one cell from its stencil Laplacians:
 // Cahn-Hilliard (src/discretization.cu:859-863) and Allen-Cahn
 // (src/discretization.cu:888-942) with the closed-form grand-potential
 // difference at the solved potential, as the quadratic form of StepParams.
+// The composition increment lap(q) is formed on its own and added to x once,
+// as the reference adds dt D lap(x_g): where the increment is ~0 (the
+// uniform matrix) x comes back unchanged, so rounding adds no drift there.
 __device__ __forceinline__ void update_cell(const StepParams& P, const double lap[4],
                                             const Point& cur, const Centre& cc, double& nCr,
                                             double& nNb, double& npd, double& npl) {
-  nCr = lap[2];
-  nNb = lap[3];
+  nCr = cc.xCr + lap[2];
+  nNb = cc.xNb + lap[3];
   const double fd = cur.pd, fl = cur.pl;
   const double m00 = cc.mu0 * cc.mu0, m01 = cc.mu0 * cc.mu1, m11 = cc.mu1 * cc.mu1;
   const double Gd = fma(P.g[0][0], m00, fma(P.g[0][1], m01, fma(P.g[0][2], m11,
@@ -390,7 +390,7 @@ __device__ __forceinline__ void march(
       }
       double lap[4];
       laplacians<NINE>(m, win[ps], W, win[cs], E, win[ns], winW[ps], winE[ps], winW[ns],
-                       winE[ns], cen[cs], lap);
+                       winE[ns], lap);
 
       double nCr, nNb, npd, npl;
       update_cell(P, lap, win[cs], cen[cs], nCr, nNb, npd, npl);
@@ -489,7 +489,7 @@ __device__ __forceinline__ void marchk(
                       Point& v, Centre& c) {
     const Point W = shift_left(C), E = shift_right(C);
     double lap[4];
-    laplacians<false>(m, Sth, W, C, E, Nth, Sth, Sth, Nth, Nth, cc, lap);
+    laplacians<false>(m, Sth, W, C, E, Nth, Sth, Sth, Nth, Nth, lap);
     double n0, n1, n2, n3;
     update_cell(P, lap, C, cc, n0, n1, n2, n3);
     Raw<T> r;
@@ -572,7 +572,7 @@ __device__ __forceinline__ void marchk(
       const Point& Nth = win[K - 1][sa];
       const Point W = shift_left(C), E = shift_right(C);
       double lap[4];
-      laplacians<false>(m, Sth, W, C, E, Nth, Sth, Sth, Nth, Nth, cen[K - 1][sc], lap);
+      laplacians<false>(m, Sth, W, C, E, Nth, Sth, Sth, Nth, Nth, lap);
       double nCr, nNb, npd, npl;
       update_cell(P, lap, C, cen[K - 1][sc], nCr, nNb, npd, npl);
       if (out_lane) {
