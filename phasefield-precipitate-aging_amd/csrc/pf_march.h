@@ -78,9 +78,11 @@ struct Raw {
   T cr, nb, pd, pl;
 };
 
-// Field access split into a wave-uniform row byte offset (kept in scalar
-// registers) and a 32-bit per-lane byte offset, so that loads and stores take
-// the scalar-base + vector-offset form with no per-lane 64-bit address math.
+// Field access as a wave-uniform row byte offset (scalar registers) plus a
+// 32-bit per-lane byte offset.  The compiler forms each address with one
+// 64-bit multiply-add of the row index onto a hoisted per-lane base; forcing
+// a scalar row base instead costs more in scalar-register spills than the
+// add it saves.
 template <typename T>
 __device__ __forceinline__ T ld_at(const T* base, size_t row_bytes, uint32_t lane_bytes) {
   return *(const T*)((const char*)base + row_bytes + lane_bytes);
