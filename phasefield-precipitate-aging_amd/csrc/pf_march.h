@@ -446,8 +446,14 @@ __device__ __forceinline__ void march(
 // row of the level's window in y.  So K steps here give the same result as K
 // one-step launches.  Five-point masks, no nucleation (the caller runs
 // nucleation steps one at a time).
-template <typename T, int BC, int K, int DEPTH, int SW = WAVE - 2 * K>
-__device__ __forceinline__ void marchk(
+//
+// IMG: whether this wave's strip and chunk can meet a no-flux image (an edge
+// strip, or a chunk within K rows of the top or bottom).  marchk() below
+// picks the variant per wave, so the ~95% of waves that meet none carry none
+// of the image handling (which the compiler would otherwise execute on every
+// wave as selects).
+template <typename T, int BC, int K, int DEPTH, int SW, bool IMG>
+__device__ __forceinline__ void marchk_impl(
     const T* __restrict__ iCr, const T* __restrict__ iNb, const T* __restrict__ ipd,
     const T* __restrict__ ipl, T* __restrict__ oCr, T* __restrict__ oNb, T* __restrict__ opd,
     T* __restrict__ opl, int Nx, int Ny, int pitch, int strip, int chunk, int rows,
@@ -463,7 +469,7 @@ __device__ __forceinline__ void marchk(
   const int px = 1 + bc_map<BC>(gx, Nx);
   const bool out_lane = (lane >= K) && (lane < K + SW) && (gx < Nx);
   // does this wave hold a no-flux image column (gx = -1 or gx = Nx)?
-  const bool x_images = BC == 0 && (strip == 0 || strip * SW + WAVE - 1 - K >= Nx);
+  const bool x_images = IMG && (strip == 0 || strip * SW + WAVE - 1 - K >= Nx);
 
   const int y0 = chunk * rows;
   const int y1 = min(y0 + rows, Ny);
@@ -525,7 +531,7 @@ __device__ __forceinline__ void marchk(
       if (l > lmax_active) break;
       level_up(win[l - 1][sb], win[l - 1][sc], cen[l - 1][sc], win[l - 1][sa], win[l][sa],
                cen[l][sa]);
-      if (BC == 0) {
+      if (IMG) {
         const int row = y0 - K + t - l;
         if (top && row == 0) {  // the image row -1 (when this chunk has one) is in slot sc
           win[l][sc] = win[l][sa];
@@ -583,6 +589,27 @@ __device__ __forceinline__ void marchk(
       }
     }
   }
+}
+
+template <typename T, int BC, int K, int DEPTH, int SW = WAVE - 2 * K>
+__device__ __forceinline__ void marchk(
+    const T* __restrict__ iCr, const T* __restrict__ iNb, const T* __restrict__ ipd,
+    const T* __restrict__ ipl, T* __restrict__ oCr, T* __restrict__ oNb, T* __restrict__ opd,
+    T* __restrict__ opl, int Nx, int Ny, int pitch, int strip, int chunk, int rows,
+    const StepParams& P) {
+  if (BC == 0) {
+    // old rows read: y0-K .. y1-1+K; lanes cover columns strip*SW-K .. strip*SW+63-K
+    const int y0 = chunk * rows, y1 = min(y0 + rows, Ny);
+    const bool edge = strip == 0 || strip * SW + WAVE - 1 - K >= Nx || y0 - K < 0 ||
+                      y1 - 1 + K >= Ny;
+    if (edge) {
+      marchk_impl<T, BC, K, DEPTH, SW, true>(iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny,
+                                             pitch, strip, chunk, rows, P);
+      return;
+    }
+  }
+  marchk_impl<T, BC, K, DEPTH, SW, false>(iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny, pitch,
+                                          strip, chunk, rows, P);
 }
 
 }  // namespace pf
