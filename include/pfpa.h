@@ -284,6 +284,14 @@ void pf_host_nucleation_probability(double xCr, double xNb, double dG, double DC
 /* Largest eigenvalue over the three phases of D * H_gamma^-1 * H_a: the
  * effective diffusivity that bounds the explicit step inside each phase. */
 double pf_host_max_effective_diffusivity(const double* DCr, const double* DNb);
+/* The per-step coefficients the step kernels run on, folded on the host from
+ * the device_evolution constants (src/discretization.cu:859-863, 888-942); no
+ * counterpart in the reference, which evaluates them term by term per cell.
+ * out22: q_Cr = dt D_Cr.(xe_g + A_g mu) as the coefficients of (mu0, mu1, 1),
+ * the same for q_Nb, then per phase (delta, Laves) the quadratic form
+ * 30 dt L dOmega = g0 mu0^2 + g1 mu0 mu1 + g2 mu1^2 + g3 mu0 + g4 mu1, then
+ * 2 omega dt L, 2 alpha dt L and kappa dt L per phase. */
+int pf_host_step_coefficients(const pf_params* prm, double* out22);
 
 #ifdef __cplusplus
 }

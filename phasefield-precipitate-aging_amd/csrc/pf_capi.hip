@@ -421,5 +421,19 @@ double pf_host_max_effective_diffusivity(const double* DCr, const double* DNb) {
   }
   return best;
 }
+int pf_host_step_coefficients(const pf_params* prm, double* out22) {
+  if (!prm || !out22) return fail(PF_ERR_ARG, "null parameters or output");
+  const pf::StepParams P = pf::make_step(prm->dt, prm->DCr, prm->DNb, prm->kappa, prm->omega,
+                                         prm->Lmob, prm->alpha, prm->mask);
+  double* o = out22;
+  for (int i = 0; i < 3; i++) *o++ = P.qCr[i];
+  for (int i = 0; i < 3; i++) *o++ = P.qNb[i];
+  for (int a = 0; a < 2; a++)
+    for (int i = 0; i < 5; i++) *o++ = P.g[a][i];
+  for (int a = 0; a < 2; a++) *o++ = P.w[a];
+  for (int a = 0; a < 2; a++) *o++ = P.c[a];
+  for (int a = 0; a < 2; a++) *o++ = P.k[a];
+  return PF_OK;
+}
 
 }  // extern "C"

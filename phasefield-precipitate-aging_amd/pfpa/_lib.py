@@ -86,6 +86,7 @@ PROTOTYPES = {
     "pf_host_xe": (_d, [_i, _i]),
     "pf_host_nucleation_probability": (None, [_d] * 10 + [_dp, _dp]),
     "pf_host_max_effective_diffusivity": (_d, [_dp, _dp]),
+    "pf_host_step_coefficients": (_i, [ctypes.POINTER(PfParams), _dp]),
 }
 
 _lib = None

@@ -165,3 +165,62 @@ def test_initial_conditions():
     # ghost ring is the periodic image
     assert np.array_equal(g[0][0, :], g[0][-2, :])
     assert np.array_equal(g[0][:, 0], g[0][:, -2])
+
+
+def test_step_coefficients_against_oracle_paraboloids(oracle):
+    """The folded coefficients the step kernels run on (pf_host_step_coefficients)
+    against the oracle's paraboloids, evaluated the reference's way:
+    x_a(mu) = xe_a + H_a^-1 mu at a common potential mu, then
+      q_Cr = dt (D_CrCr x_g,Cr + D_CrNb x_g,Nb)      (src/discretization.cu:859-863)
+      30 dt L dOmega_a = 30 dt L [g_g(x_g) - g_a(x_a) - mu.(x_g - x_a)]
+                                                    (src/discretization.cu:888-942)
+    A dropped term, a wrong sign or a missing factor 2 on the cross term fails."""
+    s = M.make_setup(64, 64)
+    from pfpa._lib import PfParams
+    P = PfParams()
+    P.dt = s.dt
+    for k in range(2):
+        P.DCr[k], P.DNb[k] = s.DCr[k], s.DNb[k]
+        P.kappa[k], P.omega[k], P.Lmob[k] = s.kappa[k], s.omega[k], s.Lmob[k]
+    P.alpha = s.alpha
+    for k in range(9):
+        P.mask[k] = s.mask[k]
+    out = (ctypes.c_double * 22)()
+    assert pfpa.lib().pf_host_step_coefficients(ctypes.byref(P), out) == 0
+    c = np.array(out[:])
+    qCr, qNb, G = c[0:3], c[3:6], c[6:16].reshape(2, 5)
+
+    def A(a):
+        H = np.array([[oracle.d2g(a, 0), oracle.d2g(a, 1)], [oracle.d2g(a, 1), oracle.d2g(a, 2)]])
+        return np.linalg.inv(H)
+
+    def xa(a, mu):
+        return np.array([oracle.xe(a, 0), oracle.xe(a, 1)]) + A(a) @ mu
+
+    rng = np.random.default_rng(3)
+    Hg = np.array([[oracle.d2g(0, 0), oracle.d2g(0, 1)], [oracle.d2g(0, 1), oracle.d2g(0, 2)]])
+    for _ in range(50):
+        # potentials of compositions across the simplex region the model visits
+        x = np.array([rng.uniform(0.0, 0.6), rng.uniform(0.0, 0.3)])
+        mu = Hg @ (x - np.array([oracle.xe(0, 0), oracle.xe(0, 1)]))
+        xg = xa(0, mu)
+        want_q = [s.dt * (s.DCr[0] * xg[0] + s.DCr[1] * xg[1]),
+                  s.dt * (s.DNb[0] * xg[0] + s.DNb[1] * xg[1])]
+        for q, w in zip((qCr, qNb), want_q):
+            assert q[0] * mu[0] + q[1] * mu[1] + q[2] == pytest.approx(w, rel=1e-12)
+        for ph in (0, 1):
+            a = 1 + ph
+            x_a = xa(a, mu)
+            dOm = oracle.g(0, *xg) - oracle.g(a, *x_a) - mu @ (xg - x_a)
+            want = 30.0 * s.dt * s.Lmob[ph] * dOm
+            got = (G[ph, 0] * mu[0] ** 2 + G[ph, 1] * mu[0] * mu[1] + G[ph, 2] * mu[1] ** 2 +
+                   G[ph, 3] * mu[0] + G[ph, 4] * mu[1])
+            scale = 30.0 * s.dt * s.Lmob[ph] * (abs(oracle.g(0, *xg)) + abs(oracle.g(a, *x_a)) +
+                                               abs(mu @ (xg - x_a)))
+            assert abs(got - want) <= 1e-12 * scale
+    for ph in (0, 1):
+        dtL = s.dt * s.Lmob[ph]
+        assert c[16 + ph] == pytest.approx(2 * s.omega[ph] * dtL, rel=1e-15)
+        assert c[18 + ph] == pytest.approx(2 * s.alpha * dtL, rel=1e-15)
+        assert c[20 + ph] == pytest.approx(s.kappa[ph] * dtL, rel=1e-15)
+    assert pfpa.lib().pf_host_step_coefficients(None, out) != 0
