@@ -121,7 +121,7 @@ __global__ __launch_bounds__(WAVE * WPB) void batch_step_kernel(const BatchJob* 
 // Nucleation on the new fields of the jobs whose schedule is due on the last
 // step of this launch (src/main.cpp:307-316 schedule, centre-only embeds),
 // then the last block advances the step counters and picks each job's next
-// advance (a two-step launch never has a nucleation step before its second
+// advance (a multi-step launch never has a nucleation step before its last
 // step).
 __host__ __device__ __forceinline__ int next_adv(const BatchJob& J) {
   int k = 1;

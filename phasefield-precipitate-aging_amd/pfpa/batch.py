@@ -1,6 +1,6 @@
 """Batched form over the C-ABI (pf_batch_*): many independent simulations of
-uneven size in one device pool, advanced by one persistent launch per step,
-optionally replayed from a captured hipGraph."""
+uneven size in one device pool, advanced by one persistent launch per one,
+two or three steps, optionally replayed from a captured hipGraph."""
 import ctypes
 
 import numpy as np
