@@ -79,6 +79,23 @@ def load_valu_issue(kernel):
     return None
 
 
+def host_cpu():
+    """Model name, logical CPUs visible and socket count of the host."""
+    model, sockets = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k == "model name" and model == "unknown":
+                    model = v.strip()
+                elif k == "physical id":
+                    sockets.add(v.strip())
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "sockets": len(sockets) or None}
+
+
 def cpu_baseline(N, steps, warmup=1):
     """Time the FP64 oracle (all host threads via OpenMP) on N x N for `steps`."""
     import oracle as O
@@ -534,7 +551,8 @@ def main():
         v, cores, el = cpu_baseline(cpu_N, args.cpu_steps)
         cpu = {"value": v, "unit": "cell-updates/s", "cores": cores, "kind": "port",
                "sample": "%d steps of the %dx%d FP64 plain step (same init, periodic) after "
-                         "1 warm-up step, %.1f s" % (args.cpu_steps, cpu_N, cpu_N, el)}
+                         "1 warm-up step, %.1f s" % (args.cpu_steps, cpu_N, cpu_N, el),
+               "host_cpu": host_cpu()}
     out["cpu_baseline"] = cpu
     out["bench_wall_s"] = time.perf_counter() - t_all
     print(json.dumps(out))
