@@ -426,7 +426,8 @@ def precision_section(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    # the 4096^2 plain workload's run length (about 0.9 s of GPU time)
+    ap.add_argument("--steps", type=int, default=10000)
     # the GPU's clock takes about 30 ms of sustained load to settle (the first
     # launches of a cold run are up to 30% slower): 600 steps are ~55 ms
     ap.add_argument("--warmup", type=int, default=600)

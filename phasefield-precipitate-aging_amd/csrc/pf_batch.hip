@@ -14,10 +14,11 @@
 // current set and writing the other: pf::marchk when the job advances two or
 // three steps this launch, pf::march for one), then the nucleation kernel
 // (jobs whose schedule is due on the last step of the launch; centre-only
-// embeds), whose last block advances every active job's step counter, retires
-// jobs that reached their target and decides each job's next advance.  A job
-// advances k steps in a launch (k up to the batch's steps per launch) when k
-// steps remain and no nucleation step falls before the last of them.
+// embeds), then a one-block kernel that advances every active job's step
+// counter, retires jobs that reached their target and decides each job's next
+// advance.  A job advances k steps in a launch (k up to the batch's steps per
+// launch) when k steps remain and no nucleation step falls before the last of
+// them.
 // A job's arithmetic and tiling depend only on the job, so its result is the
 // same bits alone or in any mix.
 #include <hip/hip_runtime.h>
@@ -52,7 +53,7 @@ struct BatchJob {
 struct BatchHeader {
   int total_waves;
   int njobs;
-  unsigned int done_blocks;      // last-block-done counter of the nucleation kernel
+  unsigned int pad;
   unsigned int nuc_counters[4];  // [0] deferred events, [1] overflow flag, [2] wide events
   int any_due;                   // some active job has a nucleation step in this launch
 };
@@ -119,10 +120,10 @@ __global__ __launch_bounds__(WAVE * WPB) void batch_step_kernel(const BatchJob* 
 }
 
 // Nucleation on the new fields of the jobs whose schedule is due on the last
-// step of this launch (src/main.cpp:307-316 schedule, centre-only embeds),
-// then the last block advances the step counters and picks each job's next
-// advance (a multi-step launch never has a nucleation step before its last
-// step).
+// step of this launch (src/main.cpp:307-316 schedule, centre-only embeds);
+// then batch_advance_kernel advances the step counters and picks each job's
+// next advance (a multi-step launch never has a nucleation step before its
+// last step).
 __host__ __device__ __forceinline__ int next_adv(const BatchJob& J) {
   int k = 1;
   while (k < J.max_adv && J.step + k + 1 <= J.target &&
@@ -132,7 +133,7 @@ __host__ __device__ __forceinline__ int next_adv(const BatchJob& J) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(WAVE * WPB) void batch_nuc_advance_kernel(
+__global__ __launch_bounds__(WAVE * WPB) void batch_nuc_kernel(
     BatchJob* __restrict__ jobs, const int* __restrict__ wave_job, BatchHeader* __restrict__ hdr,
     T* __restrict__ pool) {
   int first, last, stride, w;
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(WAVE * WPB) void batch_nuc_advance_kernel(
   const int lane = threadIdx.x & (WAVE - 1);
   // launches without a due job (all but one in nuc_every / steps per launch)
   // skip the scan of the wave table
-  if (!hdr->any_due) last = first;
+  if (!hdr->any_due) return;
   for (; w < last; w += stride) {
     const int j = __builtin_amdgcn_readfirstlane(wave_job[w]);
     const BatchJob& J = jobs[j];
@@ -175,33 +176,27 @@ __global__ __launch_bounds__(WAVE * WPB) void batch_nuc_advance_kernel(
       }
     }
   }
-  // last block done advances every job
-  __syncthreads();
-  __shared__ unsigned int ticket;
-  if (threadIdx.x == 0) {
-    __threadfence();
-    ticket = atomicAdd(&hdr->done_blocks, 1u);
-  }
-  __syncthreads();
-  if (ticket == gridDim.x - 1) {
-    __threadfence();
-    int due = 0;
-    for (int j = threadIdx.x; j < hdr->njobs; j += blockDim.x) {
-      BatchJob& J = jobs[j];
-      if (J.active) {
-        J.step += J.adv;
-        J.parity ^= 1;
-        if (J.step >= J.target) J.active = 0;
-        J.adv = next_adv(J);
-        due |= nuc_due(J) ? 1 : 0;
-      }
-    }
-    due = __syncthreads_or(due);
-    if (threadIdx.x == 0) {
-      hdr->any_due = due;
-      hdr->done_blocks = 0u;
+}
+
+// One block after the nucleation kernel (stream order): advance every active
+// job by this launch's steps, flip its buffer parity, retire it at its target,
+// decide its next advance, and record whether any job is due at the next
+// launch.
+__global__ __launch_bounds__(256) void batch_advance_kernel(BatchJob* __restrict__ jobs,
+                                                            BatchHeader* __restrict__ hdr) {
+  int due = 0;
+  for (int j = threadIdx.x; j < hdr->njobs; j += blockDim.x) {
+    BatchJob& J = jobs[j];
+    if (J.active) {
+      J.step += J.adv;
+      J.parity ^= 1;
+      if (J.step >= J.target) J.active = 0;
+      J.adv = next_adv(J);
+      due |= nuc_due(J) ? 1 : 0;
     }
   }
+  due = __syncthreads_or(due);
+  if (threadIdx.x == 0) hdr->any_due = due;
 }
 
 // per-job deterministic compensated sums: block (j, r) reduces rows
@@ -388,8 +383,11 @@ hipError_t launch_batch_step(pf_batch* b, hipStream_t st) {
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(pf::batch_nuc_advance_kernel<T>, grid, block, 0, st, b->d_jobs,
-                     b->d_wave_job, b->d_hdr, pool);
+  hipLaunchKernelGGL(pf::batch_nuc_kernel<T>, grid, block, 0, st, b->d_jobs, b->d_wave_job,
+                     b->d_hdr, pool);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(pf::batch_advance_kernel, dim3(1), dim3(256), 0, st, b->d_jobs, b->d_hdr);
   return hipGetLastError();
 }
 
@@ -623,7 +621,7 @@ int pf_batch_commit(pf_batch* b) {
     return bhip(e, "read header");
   h.total_waves = (int)wave_job.size();
   h.njobs = (int)b->jobs.size();
-  h.done_blocks = 0u;
+  h.pad = 0u;
   h.any_due = 0;
   for (size_t j = 0; j < b->jobs.size(); j++)
     if (b->slot_live[j] && pf::nuc_due(b->jobs[j])) h.any_due = 1;

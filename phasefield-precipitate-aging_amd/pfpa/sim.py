@@ -16,26 +16,31 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
-def plan_launches(t0, n, max_steps, nuc_every=0):
-    """Launches that advance steps t0 .. t0+n-1: a list of (k, nucleation)
-    with k steps per launch.  A launch groups up to `max_steps` steps none of
-    which is a nucleation step (t % nuc_every == 0, src/main.cpp:307-316);
-    a nucleation step runs alone with the nucleation pass (nucleation True).
-    nuc_every 0 = nucleation off."""
-    def due(t):
-        return nuc_every > 0 and t % nuc_every == 0
-    out = []
-    t, left = t0, n
-    while left > 0:
-        k = 1
-        for cand in (3, 2):
-            if cand <= max_steps and left >= cand and not any(due(t + j) for j in range(cand)):
-                k = cand
-                break
-        out.append((k, k == 1 and due(t)))
+def iter_launches(t0, n, max_steps, nuc_every=0):
+    """Launches that advance steps t0 .. t0+n-1, one (k, nucleation) at a
+    time with k steps per launch.  A launch groups up to `max_steps` steps
+    none of which is a nucleation step (t % nuc_every == 0,
+    src/main.cpp:307-316); a nucleation step runs alone with the nucleation
+    pass (nucleation True).  nuc_every 0 = nucleation off.  O(1) per launch,
+    so a stepping loop can start launching at once."""
+    t, end = t0, t0 + n
+    while t < end:
+        if nuc_every > 0:
+            nxt = -(-t // nuc_every) * nuc_every  # first nucleation step >= t
+            if nxt == t:
+                yield (1, True)
+                t += 1
+                continue
+            k = min(max_steps, end - t, nxt - t)
+        else:
+            k = min(max_steps, end - t)
+        yield (k, False)
         t += k
-        left -= k
-    return out
+
+
+def plan_launches(t0, n, max_steps, nuc_every=0):
+    """iter_launches as a list."""
+    return list(iter_launches(t0, n, max_steps, nuc_every))
 
 
 class Simulation:
@@ -115,7 +120,7 @@ class Simulation:
 
     def step(self, n=1):
         st = ctypes.c_void_p(self.stream.cuda_stream)
-        plan = plan_launches(self.steps_done, n, self.steps_per_launch,
+        plan = iter_launches(self.steps_done, n, self.steps_per_launch,
                              self.nuc_every if self.nucleation else 0)
         for k, nucleate in plan:
             a, b = self.a, self.b

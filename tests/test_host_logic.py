@@ -36,6 +36,33 @@ def test_plan_launches(t0, max_steps, nuc_every):
             t += k
 
 
+def _plan_brute_force(t0, n, max_steps, nuc_every):
+    """Per launch, the longest group of 3 or 2 steps with no nucleation step in
+    it, else one step (with the pass when it is a nucleation step)."""
+    def due(t):
+        return nuc_every > 0 and t % nuc_every == 0
+    out, t, left = [], t0, n
+    while left > 0:
+        k = 1
+        for cand in (3, 2):
+            if cand <= max_steps and left >= cand and not any(due(t + j) for j in range(cand)):
+                k = cand
+                break
+        out.append((k, k == 1 and due(t)))
+        t += k
+        left -= k
+    return out
+
+
+@pytest.mark.parametrize("max_steps", [1, 2, 3])
+def test_plan_launches_against_brute_force(max_steps):
+    for nuc_every in (0, 1, 2, 3, 4, 5, 7, 891):
+        for t0 in (0, 1, 2, 5, 889, 890, 891):
+            for n in (0, 1, 2, 3, 4, 7, 50, 2000):
+                assert plan_launches(t0, n, max_steps, nuc_every) == \
+                    _plan_brute_force(t0, n, max_steps, nuc_every), (t0, n, nuc_every)
+
+
 def test_plan_launches_schedule_of_the_benchmark():
     """Nucleation every 891 steps from step 0: 1000 steps are one nucleation
     launch, three-step launches and the groups the schedule leaves."""
