@@ -232,7 +232,10 @@ def batched_section(args):
     pool = sum(job_bytes(PF_STORAGE_F32, sp["n"], sp["n"]) for sp in specs) + (64 << 20)
     batch = Batch(PF_STORAGE_F32, pool_bytes=pool, max_jobs=len(specs))
     total_steps = args.batch_steps
-    target = total_steps + 1000      # room for the autotuner's launches
+    # room for the autotuner's launches (at most 3 steps x 10 launches x 45
+    # configurations, then the graph candidates), so that every job is still
+    # active through the timed run
+    target = total_steps + 2000
     plain_inputs = []
     t_gen = time.perf_counter()
     for sp in specs:

@@ -59,7 +59,7 @@ def step_counts(batch):
     return {j: steps for j, steps, _ in batch.poll_all()}
 
 
-def autotune_batch(batch, launches=8, rows_set=(8, 16, 32), depth_set=(2, 3, 6),
+def autotune_batch(batch, launches=8, rows_set=(8, 16, 32, 48, 64), depth_set=(2, 3, 6),
                    launch_steps_set=(1, 2, 3), graph_set=(1, 8)):
     """Measure the current job mix's cell-updates/s for each tile shape
     (rows, prefetch depth), time steps per launch (one, or two with the
