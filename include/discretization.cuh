@@ -70,7 +70,10 @@ void device_seed_prng(struct CudaData* dev, uint64_t seed);
  * Code that reads the device arrays by other means calls device_flush first. */
 void device_set_steps_per_launch(struct CudaData* dev, int steps);
 
-/** Run the steps device_evolution has deferred (see above). */
+/** Queue the steps device_evolution has deferred (see above).  The steps run
+ * on the API's stream, which is a blocking stream as in the reference: work
+ * the caller then issues on the default stream (a hipMemcpy of the device
+ * arrays) runs after them. */
 void device_flush(struct CudaData* dev);
 
 #endif /* _CUDA_KERNELS_H_ */

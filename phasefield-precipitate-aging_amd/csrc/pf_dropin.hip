@@ -125,7 +125,10 @@ void init_cuda(struct HostData* host, const int nx, const int ny, const int nm,
   const size_t bytes = (size_t)nx * ny * sizeof(fp_t);
   dev->prng.seed = 42;
   dev->prng.calls = 0;
-  (void)checkCuda(hipStreamCreateWithFlags(&dev->str_A, hipStreamNonBlocking));
+  // a blocking stream, as the reference's cudaStreamCreate (src/data.cu:19-22):
+  // a caller's work on the default stream (a hipMemcpy of the device arrays)
+  // is ordered after the steps queued here
+  (void)checkCuda(hipStreamCreate(&dev->str_A));
   dev->str_B = dev->str_C = dev->str_D = dev->str_A;
   (void)checkCuda(hipEventCreateWithFlags(&dev->ev_A, hipEventDisableTiming));
   dev->ev_B = dev->ev_C = dev->ev_D = dev->ev_A;

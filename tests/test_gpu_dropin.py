@@ -135,3 +135,92 @@ def test_python_api_replay(gpu, oracle, spl):
     ws = oracle.summarize(want)
     for a, b in zip(summary, ws):
         assert a == pytest.approx(b, rel=1e-12, abs=1e-15)
+
+
+def _grouping_input(Nx, Ny, bc, seed):
+    s = M.make_setup(Nx, Ny, bc=bc)
+    f0 = M.init_matrix_with_particles(Nx, Ny, s.dx, bc, seed=seed, radius=2e-9)
+    return s, f0
+
+
+def test_cpp_grouping_edge_cases(gpu):
+    """tests/cpp/grouping_main: a dt change inside a group, a step whose
+    pointers were not swapped, a nucleation pass with events, a direct device
+    read after device_flush; grouped == one launch per call, bit for bit."""
+    exe = os.path.join(ROOT, "tests", "cpp", "grouping_main")
+    assert os.path.exists(exe), "tests/cpp/grouping_main not built: run __graft_entry__.build()"
+    Nx, Ny = 90, 64
+    s, f0 = _grouping_input(Nx, Ny, 0, 12)
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        with open(fin, "wb") as fh:
+            fh.write(struct.pack("ii", Nx + 2, Ny + 2))
+            for a in f0:
+                fh.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+        r = subprocess.run([exe, fin, repr(s.dt), repr(s.dx)], capture_output=True, text=True,
+                           timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "grouping ok" in r.stdout
+
+
+def _python_grouping_sequence(spl, s, f0):
+    import torch
+    Nx, Ny = s.N
+    nx, ny, nm, bx, by = Nx + 2, Ny + 2, 3, 13, 13
+    host, dev = api.HostData(), api.CudaData()
+    api.make_arrays(host, nx, ny, nm)
+    api.five_point_Laplacian_stencil(s.dx, s.dx, host.mask_lap, nm)
+    for name, a in zip(("conc_Cr_old", "conc_Nb_old", "phi_del_old", "phi_lav_old"), f0):
+        getattr(host, name)[...] = a
+    D_Cr, D_Nb, Lmob = M.set_diffusion_matrix(0.30, 0.02)
+    api.init_cuda(host, nx, ny, nm, M.kappa(), M.omega(), Lmob, D_Cr, D_Nb, dev)
+    api.device_init_prng(dev, nx, ny, nm, bx, by, seed=7)
+    api.device_set_steps_per_launch(dev, spl)
+
+    def swap():
+        dev.conc_Cr_old, dev.conc_Cr_new = api.swap_pointers_1D(dev.conc_Cr_old, dev.conc_Cr_new)
+        dev.conc_Nb_old, dev.conc_Nb_new = api.swap_pointers_1D(dev.conc_Nb_old, dev.conc_Nb_new)
+        dev.phi_del_old, dev.phi_del_new = api.swap_pointers_1D(dev.phi_del_old, dev.phi_del_new)
+        dev.phi_lav_old, dev.phi_lav_new = api.swap_pointers_1D(dev.phi_lav_old, dev.phi_lav_new)
+
+    def step(h, do_swap=True):
+        api.device_evolution(dev, nx, ny, nm, bx, by, M.OMEGA_3PHASE, h)
+        if do_swap:
+            swap()
+
+    dt = s.dt
+    step(dt)
+    step(dt)
+    step(0.5 * dt)              # dt changes inside the group
+    step(0.5 * dt)
+    step(0.5 * dt)
+    step(dt, do_swap=False)     # the chain breaks: old -> new twice
+    step(dt)
+    step(dt)
+    step(dt)
+    api.device_evolution(dev, nx, ny, nm, bx, by, M.OMEGA_3PHASE, dt)
+    api.device_nucleation(dev, nx, ny, nm, bx, by, 0.0105, 0.0108, M.LATTICE_CONST,
+                          M.IFCE_WIDTH, s.dx, s.dx, s.dx, 3e38)
+    swap()
+    step(dt)
+    step(dt)
+    api.device_flush(dev)
+    torch.cuda.synchronize()
+    mid = dev.conc_Cr_old.cpu().numpy().copy()
+    for _ in range(8):
+        step(dt)
+    api.read_out_result(dev, host, nx, ny)
+    out = [host.conc_Cr_new.copy(), host.conc_Nb_new.copy(), host.phi_del_new.copy(),
+           host.phi_lav_new.copy()]
+    api.free_cuda(dev)
+    return mid, out
+
+
+def test_python_grouping_edge_cases(gpu):
+    """The same sequence through the Python mirror (pfpa.api)."""
+    s, f0 = _grouping_input(90, 64, 0, 12)
+    m3, o3 = _python_grouping_sequence(3, s, f0)
+    m1, o1 = _python_grouping_sequence(1, s, f0)
+    assert np.array_equal(m3, m1)
+    for a, b in zip(o3, o1):
+        assert np.array_equal(a, b)
