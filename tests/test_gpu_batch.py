@@ -185,3 +185,46 @@ def test_batch_summarize(gpu, oracle):
     for g, w in zip(got, want):
         assert g == pytest.approx(w, rel=1e-14, abs=1e-16)
     b.close()
+
+
+def test_batch_errors(gpu):
+    """Invalid arguments are rejected with an error and leave the batch usable."""
+    from pfpa import PfError
+    with pytest.raises(PfError):
+        Batch(PF_STORAGE_F64, pool_bytes=1000)            # pool under 1 MiB
+    with pytest.raises(PfError):
+        Batch(7, pool_bytes=64 << 20)                     # unknown storage
+    b = Batch(PF_STORAGE_F64, pool_bytes=64 << 20, max_jobs=2)
+    for bad in ((0, 3), (8, 4), (1 << 17, 3)):
+        with pytest.raises(PfError):
+            b.set_tile(*bad)
+    for bad in (0, 4):
+        with pytest.raises(PfError):
+            b.set_launch_steps(bad)
+    s = M.make_setup(40, 30, bc=0)
+    f = rand_fields(40, 30, 0, 1)
+    d = make_desc(s, 5)
+    d.bc = 7
+    with pytest.raises(PfError):
+        b.add(d, f)
+    d = make_desc(s, 5)
+    d.prm.mask[0] = 1.0                                   # a nine-point mask
+    with pytest.raises(PfError):
+        b.add(d, f)
+    with pytest.raises(ValueError):
+        b.add(make_desc(s, 5), rand_fields(41, 30, 0, 1))  # fields of another size
+    j0 = b.add(make_desc(s, 5), f)
+    j1 = b.add(make_desc(s, 7), f)
+    with pytest.raises(PfError):
+        b.add(make_desc(s, 5), f)                         # job table full
+    for call in (lambda: b.remove(999), lambda: b.poll(999), lambda: b.summarize(999),
+                 lambda: b.run(-1)):
+        with pytest.raises(PfError):
+            call()
+    b.run(8)
+    assert b.poll(j0) == (5, False) and b.poll(j1) == (7, False)
+    want = plain(s, f, 7, torch.float64)
+    got = b.read(j1)
+    for k in range(4):
+        assert np.array_equal(got[k], want[k]), k
+    b.close()
