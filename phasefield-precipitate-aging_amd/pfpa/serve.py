@@ -31,6 +31,7 @@ class Request:
     done: float = -1.0
     summary: list = field(default_factory=list)
     job: int = -1
+    fields: list = None     # final fields, when serve(read_fields=True)
 
 
 def make_requests(rate, duration, seed=1, n_lo=128, n_hi=2048, steps_lo=500, steps_hi=20000,
@@ -71,10 +72,11 @@ def request_desc(r, dx=0.25e-9):
 
 
 def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_steps=3,
-          log=None, storage=PF_STORAGE_F32, time_limit=None):
-    """Run the request stream; returns (requests, stats).  Each device round is
-    one replay of a graph of `launches_per_graph` step launches, each advancing
-    every running job by `launch_steps` steps (1 or 2)."""
+          log=None, storage=PF_STORAGE_F32, time_limit=None, read_fields=False):
+    """Run the request stream; returns (finished requests, stats).  Each device
+    round is one replay of a graph of `launches_per_graph` step launches, each
+    advancing every running job by up to `launch_steps` steps (1, 2 or 3).
+    read_fields: also keep each request's final fields (r.fields)."""
     batch = Batch(storage, pool_bytes=pool_bytes, max_jobs=max(16, len(requests)))
     batch.set_tile(rows, depth)
     batch.set_launch_steps(launch_steps)
@@ -123,6 +125,8 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
                 if r is None or active:
                     continue
                 r.summary = batch.summarize(jid)
+                if read_fields:
+                    r.fields = batch.read(jid)
                 r.done = time.perf_counter() - t0
                 cell_updates += r.n * r.n * r.steps
                 batch.remove(jid)

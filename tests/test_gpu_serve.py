@@ -1,0 +1,42 @@
+"""The serving loop (pfpa.serve): requests admitted by HBM budget with
+continuous batching give the same bits as each request served alone and as
+the plain FP32-storage path run alone."""
+import numpy as np
+import pytest
+import torch
+
+from pfpa.batch import PF_STORAGE_F32, job_bytes
+from pfpa.serve import make_requests, request_desc, request_fields, serve
+from pfpa.sim import Simulation
+
+pytestmark = pytest.mark.gpu
+
+
+def test_serving_alone_vs_mix():
+    reqs = make_requests(rate=400.0, duration=0.03, seed=5, n_lo=64, n_hi=150, steps_lo=20,
+                         steps_hi=70)
+    assert len(reqs) >= 6
+    # room for about three of the largest at once: later arrivals wait and
+    # are admitted as earlier ones retire
+    pool = 3 * job_bytes(PF_STORAGE_F32, 150, 150) + (1 << 20)
+    mixed, stats = serve(reqs, pool, launches_per_graph=5, rows=16, read_fields=True)
+    assert stats["completed"] == len(reqs)
+    assert max(r.admitted for r in mixed) > min(r.done for r in mixed)   # admissions overlap
+    by_id = {r.rid: r for r in mixed}
+    for rid in (0, len(reqs) // 2, len(reqs) - 1):
+        r0 = reqs[rid]
+        alone, _ = serve([type(r0)(r0.rid, 0.0, r0.n, r0.steps, r0.x0, r0.seed)], pool,
+                         launches_per_graph=5, rows=16, read_fields=True)
+        a, m = alone[0], by_id[rid]
+        for k in range(4):
+            assert np.array_equal(a.fields[k], m.fields[k]), (rid, k)
+        assert a.summary == m.summary
+        # and the plain FP32-storage path alone
+        s, _ = request_desc(r0)
+        sim = Simulation(s, request_fields(r0), dtype=torch.float32, nucleation=True,
+                         seed=r0.seed, instance=r0.rid)
+        sim.step(r0.steps)
+        want = sim.fields()
+        sim.close()
+        for k in range(4):
+            assert np.array_equal(m.fields[k].astype(np.float64), want[k]), (rid, k)
