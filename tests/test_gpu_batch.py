@@ -228,3 +228,29 @@ def test_batch_errors(gpu):
     for k in range(4):
         assert np.array_equal(got[k], want[k]), k
     b.close()
+
+
+def test_autotuner_leaves_a_consistent_batch(gpu):
+    """autotune_batch over a small grid of candidates: the best entry is one of
+    the measured ones, the batch is left on it, and the jobs' results after
+    tuning and a further run still equal the plain path at their step counts."""
+    from pfpa.workloads import autotune_batch
+    b = Batch(PF_STORAGE_F32, pool_bytes=128 << 20)
+    setups = []
+    for k, (Nx, Ny, bc) in enumerate(JOBS[:4]):
+        s = M.make_setup(Nx, Ny, bc=bc)
+        f = rand_fields(Nx, Ny, bc, 300 + k)
+        setups.append((b.add(make_desc(s, 400, instance=k), f), s, f))
+    best, table = autotune_batch(b, launches=2, rows_set=(8, 32), depth_set=(3,),
+                                 launch_steps_set=(1, 3), graph_set=(1, 4))
+    assert best in table and best[0] == max(t[0] for t in table)
+    assert len(table) == 2 * 2 + 1
+    b.run(3, best[4] if best[4] > 1 else 0)
+    for jid, s, f in setups:
+        steps, active = b.poll(jid)
+        assert active and 0 < steps < 400
+        want = plain(s, f, steps, torch.float32)
+        got = b.read(jid)
+        for k in range(4):
+            assert np.array_equal(got[k].astype(np.float64), want[k]), (jid, k)
+    b.close()
