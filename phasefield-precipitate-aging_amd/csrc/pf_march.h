@@ -6,9 +6,9 @@
 // One wave64 owns a strip of 64 columns (62 output columns + one halo column
 // each side) and marches down a chunk of rows.  Each lane keeps a three-row
 // window (y-1, y, y+1) of what the stencil needs -- phi_delta, phi_Laves and
-// the composition potentials q = dt D x_gamma -- in registers, so each cell is loaded
-// once per strip and its KKS solve is done once.  x-neighbours come from
-// adjacent lanes through DPP wave shifts (no LDS round trip).  Loads run DEPTH
+// the composition potentials q = dt D x_gamma -- in registers, so each cell
+// is loaded once per strip and its KKS solve is done once.  x-neighbours come
+// from adjacent lanes through DPP wave shifts (no LDS round trip).  Loads run DEPTH
 // rows ahead of the row being computed (a register ring), so a wave keeps
 // several rows of HBM requests in flight; the row loop is unrolled by 6 so the
 // window and the ring rotate by register renaming instead of moves.
