@@ -380,8 +380,8 @@ def serving_section(args):
     from pfpa.serve import fp64_replay_error, make_requests, serve
     reqs = make_requests(args.serve_rate, args.serve_duration, seed=1)
     finished, stats = serve(reqs, pool_bytes=args.serve_pool_gb << 30,
-                            launches_per_graph=args.serve_lpg, rows=16, depth=3,
-                            launch_steps=3)
+                            launches_per_graph=args.serve_lpg, rows=args.serve_rows,
+                            depth=3, launch_steps=3)
     sample = sorted(finished, key=lambda r: r.n * r.n * r.steps)[:3]
     err, _ = fp64_replay_error(finished, sample)
     stats["arrival_rate_per_s"] = args.serve_rate
@@ -452,6 +452,8 @@ def main():
     ap.add_argument("--serve-pool-gb", type=int, default=4)
     ap.add_argument("--serve-lpg", type=int, default=17,
                     help="step launches per captured graph in the serving loop")
+    ap.add_argument("--serve-rows", type=int, default=16,
+                    help="rows per strip task of the serving batch")
     ap.add_argument("--precision-marks", type=int, nargs="+", default=[1000, 10000, 20000])
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the one-, two- and three-step kernels (FP64, FP32) at --size "

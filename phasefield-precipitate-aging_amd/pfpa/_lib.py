@@ -99,6 +99,11 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise PfError("libpfpa.so not built (%s): run __graft_entry__.build() or "
                           "python phasefield-precipitate-aging_amd/build.py" % LIB_PATH)
+        # PyTorch ships its own HIP runtime (same soname as /opt/rocm's).  Load
+        # it first, so that libpfpa binds to that one runtime: loaded the other
+        # way round the process would hold two, and torch could not see the GPU
+        # or share streams and allocations with the library.
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in PROTOTYPES.items():
             f = getattr(L, name)
