@@ -66,6 +66,16 @@ def five_point_mask(dx, dy):
     return m
 
 
+def nine_point_mask(dx, dy):
+    """src/mesh.cpp:198-213 as a row-major 3x3 list (the same expressions, so
+    the same doubles as the C++ set_mask(dx, dy, 93, ...))."""
+    c = 1.0 / (6.0 * dx * dy)
+    return [c, 4.0 / (6.0 * dy * dy), c,
+            4.0 / (6.0 * dx * dx), -10.0 * (dx * dx + dy * dy) / (6.0 * dx * dx * dy * dy),
+            4.0 / (6.0 * dx * dx),
+            c, 4.0 / (6.0 * dy * dy), c]
+
+
 def max_effective_diffusivity(DCr, DNb):
     a = (ctypes.c_double * 2)(*DCr)
     b = (ctypes.c_double * 2)(*DNb)

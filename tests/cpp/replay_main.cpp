@@ -6,12 +6,14 @@
 // swap_pointers_1D x4, and on I/O steps device_dataviz + read_out_result.
 //
 //   replay_main <in.bin> <out.bin> <steps> <dt> <dx> <nuc_interval> <seed> <io_interval>
-//               [steps_per_launch]
+//               [steps_per_launch [bc [mask_code]]]
 //
 // in.bin / out.bin: int32 nx, int32 ny, then conc_Cr, conc_Nb, phi_del,
 // phi_lav as nx*ny doubles each (ghost ring included).  nuc_interval 0 = off.
-// steps_per_launch (1 or 3, default: the library's) is the drop-in's grouping
-// of device_evolution calls.
+// steps_per_launch (1 or 3, 0 = the library's default) is the drop-in's
+// grouping of device_evolution calls; bc 0 = no-flux (the reference's), 1 =
+// periodic (device_set_boundary); mask_code 53 = five-point, 93 = nine-point
+// (set_mask, src/mesh.cpp:170-185).
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -25,14 +27,16 @@
 #include "pfpa.h"
 
 int main(int argc, char* argv[]) {
-  if (argc != 9 && argc != 10) {
+  if (argc < 9 || argc > 12) {
     fprintf(stderr,
             "usage: %s in.bin out.bin steps dt dx nuc_interval seed io_interval "
-            "[steps_per_launch]\n",
+            "[steps_per_launch [bc [mask_code]]]\n",
             argv[0]);
     return 2;
   }
-  const int steps_per_launch = argc == 10 ? atoi(argv[9]) : 0;
+  const int steps_per_launch = argc >= 10 ? atoi(argv[9]) : 0;
+  const int bc = argc >= 11 ? atoi(argv[10]) : 0;
+  const int mask_code = argc >= 12 ? atoi(argv[11]) : 53;
   const long steps = atol(argv[3]);
   const double dt = atof(argv[4]);
   const double meshres = atof(argv[5]);
@@ -51,7 +55,10 @@ int main(int argc, char* argv[]) {
   struct CudaData dev;
   const int bx = 13, by = 13, nm = 3;
   make_arrays(&host, nx, ny, nm);
-  five_point_Laplacian_stencil(meshres, meshres, host.mask_lap, nm);
+  if (argc >= 12)
+    set_mask(meshres, meshres, mask_code, host.mask_lap, nm);
+  else
+    five_point_Laplacian_stencil(meshres, meshres, host.mask_lap, nm);
   fp_t** fields[4] = {host.conc_Cr_old, host.conc_Nb_old, host.phi_del_old, host.phi_lav_old};
   for (int k = 0; k < 4; k++)
     if (fread(fields[k][0], sizeof(double), (size_t)nx * ny, fi) != (size_t)nx * ny) return 3;
@@ -76,6 +83,7 @@ int main(int argc, char* argv[]) {
   device_init_prng(&dev, nx, ny, nm, bx, by);
   device_seed_prng(&dev, seed);
   if (steps_per_launch) device_set_steps_per_launch(&dev, steps_per_launch);
+  if (bc) device_set_boundary(&dev, bc);
 
   // the last two thirds of the steps are timed (host wall clock from a point
   // where the device has finished the first third until it has finished all),

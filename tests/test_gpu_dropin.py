@@ -32,7 +32,7 @@ def oracle_reference(oracle, f0, setup, steps):
     return oracle.run(f0, P, 0, steps)
 
 
-def run_harness(f0, s, steps, nuc, io, spl=None):
+def run_harness(f0, s, steps, nuc, io, spl=None, bc=None, mask_code=None):
     Nx, Ny = s.N
     with tempfile.TemporaryDirectory() as td:
         fin = os.path.join(td, "in.bin")
@@ -42,8 +42,12 @@ def run_harness(f0, s, steps, nuc, io, spl=None):
             for a in f0:
                 fh.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
         args = [HARNESS, fin, fout, str(steps), repr(s.dt), repr(s.dx), str(nuc), "42", str(io)]
-        if spl is not None:
-            args.append(str(spl))
+        if spl is not None or bc is not None:
+            args.append(str(spl or 0))
+        if bc is not None:
+            args.append(str(bc))
+            if mask_code is not None:
+                args.append(str(mask_code))
         r = subprocess.run(args, capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stderr
         raw = np.fromfile(fout, dtype=np.uint8)
@@ -71,6 +75,26 @@ def test_cpp_driver_replay(gpu, oracle):
     p = lambda x: x ** 3 * (6 * x * x - 15 * x + 10)  # noqa: E731
     assert np.allclose(arr[5], p(arr[2]) + p(arr[3]), rtol=1e-13, atol=1e-15)
     assert (Nx + 2) * (Ny + 2) == arr[0].size
+
+
+@pytest.mark.parametrize("bc,mask_code", [(1, 53), (1, 93), (0, 93)])
+def test_cpp_driver_replay_boundary_and_mask(gpu, oracle, bc, mask_code):
+    """device_set_boundary (periodic) and set_mask(.., 93, ..) (the nine-point
+    mask, one launch per call) through the C++ API: grouped == one launch per
+    call, and both match the oracle with the same mask."""
+    Nx, Ny = 96, 72
+    s = M.make_setup(Nx, Ny, bc=bc, dt_rule=5e-8)
+    if mask_code == 93:
+        s.mask = M.nine_point_mask(s.dx, s.dx)
+    f0 = M.init_matrix_with_particles(Nx, Ny, s.dx, bc, seed=13, radius=2e-9)
+    steps = 40
+    a = run_harness(f0, s, steps, 0, 0, spl=3, bc=bc, mask_code=mask_code)
+    b = run_harness(f0, s, steps, 0, 0, spl=1, bc=bc, mask_code=mask_code)
+    P = oracle.make_params(s.dt, s.DCr, s.DNb, s.kappa, s.omega, s.Lmob, s.alpha, s.mask)
+    want = oracle.run(f0, P, bc, steps)
+    for k in range(4):
+        assert np.array_equal(a[k], b[k]), k
+        assert mixed_err(a[k], want[k]) <= TOL, k
 
 
 @pytest.mark.parametrize("io", [1, 2, 7])
