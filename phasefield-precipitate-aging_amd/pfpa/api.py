@@ -3,7 +3,8 @@
 Same names, argument order and meaning as the C++ functions the driver calls
 (src/main.cpp:236-321):
 
-    make_arrays, five_point_Laplacian_stencil            src/mesh.h:20-79
+    make_arrays, five_point_Laplacian_stencil,
+    nine_point_Laplacian_stencil, set_mask               src/mesh.h:20-86
     init_cuda, read_out_result, free_cuda               src/data.cuh:16-35
     device_init_prng, device_boundaries, device_fictitious, fictitious_boundaries,
     device_laplacian, device_laplacian_boundaries, device_evolution,
@@ -36,6 +37,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from . import model as M
 from ._lib import PfNucParams, PfParams, check, lib
 
 DEFAULT_SEED = 42
@@ -109,6 +111,22 @@ def five_point_Laplacian_stencil(dx, dy, mask_lap, nm):
     mask_lap[1, 1] = -2.0 * (dx * dx + dy * dy) / (dx * dx * dy * dy)
     mask_lap[1, 2] = 1.0 / (dx * dx)
     mask_lap[2, 1] = 1.0 / (dy * dy)
+
+
+def nine_point_Laplacian_stencil(dx, dy, mask_lap, nm):
+    """src/mesh.cpp:198-213 (fills mask_lap in place)."""
+    if nm != 3:
+        raise ValueError("nine_point_Laplacian_stencil requires nm == 3")
+    mask_lap[:, :] = np.array(M.nine_point_mask(dx, dy)).reshape(3, 3)
+
+
+def set_mask(dx, dy, code, mask_lap, nm):
+    """src/mesh.cpp:170-185: 93 selects the nine-point mask, anything else the
+    five-point one."""
+    if code == 93:
+        nine_point_Laplacian_stencil(dx, dy, mask_lap, nm)
+    else:
+        five_point_Laplacian_stencil(dx, dy, mask_lap, nm)
 
 
 def swap_pointers_1D(a, b):

@@ -224,3 +224,28 @@ def test_step_coefficients_against_oracle_paraboloids(oracle):
         assert c[18 + ph] == pytest.approx(2 * s.alpha * dtL, rel=1e-15)
         assert c[20 + ph] == pytest.approx(s.kappa[ph] * dtL, rel=1e-15)
     assert pfpa.lib().pf_host_step_coefficients(None, out) != 0
+
+
+@pytest.mark.parametrize("code", [53, 93, 0])
+def test_cpp_set_mask_matches_python_mirror(code):
+    """The C++ drop-in's set_mask (src/mesh.cpp:170-213 restated) and the
+    Python mirror's build the same doubles, for square and oblong cells; the
+    nine-point mask of oblong cells does not sum to zero (as the reference's)."""
+    from pfpa import api
+    fn = getattr(pfpa.lib(), "_Z8set_maskddiPPdi")
+    fn.restype = None
+    fn.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                   ctypes.POINTER(ctypes.POINTER(ctypes.c_double)), ctypes.c_int]
+    for dx, dy in ((0.25e-9, 0.25e-9), (0.25e-9, 0.325e-9)):
+        rows = [(ctypes.c_double * 3)() for _ in range(3)]
+        ptrs = (ctypes.POINTER(ctypes.c_double) * 3)(
+            *[ctypes.cast(r, ctypes.POINTER(ctypes.c_double)) for r in rows])
+        fn(dx, dy, code, ptrs, 3)
+        got = [rows[i][j] for i in range(3) for j in range(3)]
+        m = np.zeros((3, 3))
+        api.set_mask(dx, dy, code, m, 3)
+        assert got == m.ravel().tolist()
+        if code == 93 and dx != dy:
+            assert abs(sum(got)) > 1e-3 * max(abs(v) for v in got)
+        else:
+            assert abs(sum(got)) <= 1e-12 * max(abs(v) for v in got)
