@@ -356,6 +356,35 @@ def test_mass_conservation_4096(gpu):
     sim.close()
 
 
+def test_precision_form_8192(gpu):
+    """The precision workload's size (8192^2, the plain workload's
+    initialisation, periodic): FP32 storage and FP64 from the same fields for
+    30 steps (ten three-step launches).  FP64 conserves each composition sum to
+    rounding; FP32 storage stays within a few FP32 ulps of x per step of FP64
+    and conserves the sums to FP32 rounding."""
+    N = 8192
+    s = M.make_setup(N, N, bc=1)
+    f0 = M.init_enriched_band(N, N, 1, seed=42)
+    s64 = Simulation(s, f0, dtype=torch.float64)
+    s32 = Simulation(s, f0, dtype=torch.float32)
+    del f0
+    m64, m32 = s64.summarize(), s32.summarize()
+    s64.step(30)
+    s32.step(30)
+    e64, e32 = s64.summarize(), s32.summarize()
+    for k in range(2):
+        assert e64[k] == pytest.approx(m64[k], rel=1e-13)
+        assert e32[k] == pytest.approx(m32[k], rel=1e-7)
+    worst = 0.0
+    for k in range(2):
+        a = s64.a[k].reshape(s.ny, s.nx)[1:-1, 1:-1]
+        b = s32.a[k].reshape(s.ny, s.nx)[1:-1, 1:-1].double()
+        worst = max(worst, float((a - b).abs().max()))
+    assert worst < 2e-6, worst
+    s64.close()
+    s32.close()
+
+
 def test_errors_are_reported(gpu):
     L = lib()
     ctx = ctypes.c_void_p()
