@@ -407,9 +407,20 @@ def precision_section(args):
     m0 = s64.summarize()
     rows = []
     done = 0
+    secs = {64: 0.0, 32: 0.0}   # GPU time of each form's stepping (HIP events)
+
+    def timed_steps(sim, n, key):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(sim.stream)
+        sim.step(n)
+        e1.record(sim.stream)
+        e1.synchronize()
+        secs[key] += e0.elapsed_time(e1) / 1e3
+
     for mark in args.precision_marks:
-        s64.step(mark - done)
-        s32.step(mark - done)
+        timed_steps(s64, mark - done, 64)
+        timed_steps(s32, mark - done, 32)
         done = mark
         dCr = float((s64.a[0] - s32.a[0].double()).abs().max())
         dNb = float((s64.a[1] - s32.a[1].double()).abs().max())
@@ -423,7 +434,10 @@ def precision_section(args):
         log("precision", json.dumps(rows[-1]))
     s64.close()
     s32.close()
-    return rows
+    rates = {"fp64": N * N * done / secs[64], "fp32_storage": N * N * done / secs[32],
+             "unit": "cell-updates/s", "steps": done}
+    log("precision rates", json.dumps(rates))
+    return rows, rates
 
 
 def main():
@@ -512,8 +526,9 @@ def main():
 
     if args.form == "precision":
         args.size = args.size or 8192
-        rows = precision_section(args)
-        print(json.dumps({"form": "precision", "grid": [args.size, args.size], "rows": rows}))
+        rows, rates = precision_section(args)
+        print(json.dumps({"form": "precision", "grid": [args.size, args.size], "rows": rows,
+                          "rates": rates}))
         return
 
     args.size = args.size or 4096
