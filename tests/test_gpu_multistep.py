@@ -154,3 +154,53 @@ def test_multistep_errors(ctx, k):
     assert lib().pf_ctx_set_tile2(ctx, 8, 4) != 0
     assert two(ctx, 10, 10, 0, ctypes.byref(P), *map(_p, f[:4]), *map(_p, f[4:]), None) == 0
     torch.cuda.synchronize()
+
+
+def test_step_loops_equal_single_calls(gpu):
+    """pf_step_loop_f64 / pf_step2_loop_f64 / pf_step3_loop_f64 (one launch per
+    grid of a list, the batched form's comparison baseline) give the same bits
+    as the single-grid entry points on each grid; a bad list is rejected."""
+    from pfpa._lib import PfParams, PfStepItem, check, lib
+    grids = [(40, 30, 0), (129, 17, 1), (7, 64, 1)]
+    items = (PfStepItem * len(grids))()
+    keep = []
+    singles = []
+    for i, (Nx, Ny, bc) in enumerate(grids):
+        s = M.make_setup(Nx, Ny, bc=bc, dt_rule=4e-7)
+        f = _fields(Nx, Ny, bc, 70 + i)
+        P = PfParams()
+        P.dt = s.dt
+        for k in range(2):
+            P.DCr[k], P.DNb[k] = s.DCr[k], s.DNb[k]
+            P.kappa[k], P.omega[k], P.Lmob[k] = s.kappa[k], s.omega[k], s.Lmob[k]
+        P.alpha = s.alpha
+        for k in range(9):
+            P.mask[k] = s.mask[k]
+        A = [torch.as_tensor(x).cuda() for x in f]
+        B = [torch.empty_like(x) for x in A]
+        it = items[i]
+        it.nx, it.ny, it.bc = s.nx, s.ny, s.bc
+        it.prm = ctypes.pointer(P)
+        for k in range(4):
+            it.old_fields[k] = A[k].data_ptr()
+            it.new_fields[k] = B[k].data_ptr()
+        keep.append((P, A, B))
+        singles.append((s, P, A))
+    ctx = ctypes.c_void_p()
+    check(lib().pf_ctx_create(ctypes.byref(ctx)), "ctx")
+    for name, single in (("pf_step_loop_f64", "pf_step_f64"), ("pf_step2_loop_f64", "pf_step2_f64"),
+                         ("pf_step3_loop_f64", "pf_step3_f64")):
+        check(getattr(lib(), name)(ctx, len(grids), items, None), name)
+        torch.cuda.synchronize()
+        for (s, P, A), (_, _, B) in zip(singles, keep):
+            C = [torch.empty_like(x) for x in A]
+            args = [ctx, s.nx, s.ny, s.bc, ctypes.byref(P), *[_p(x) for x in A], *[_p(x) for x in C]]
+            if single == "pf_step_f64":
+                args.append(None)
+            check(getattr(lib(), single)(*args, None), single)
+            torch.cuda.synchronize()
+            for k in range(4):
+                assert torch.equal(B[k], C[k]), (name, s.N, k)
+        assert getattr(lib(), name)(ctx, -1, items, None) != 0
+        assert getattr(lib(), name)(ctx, 2, None, None) != 0
+    lib().pf_ctx_destroy(ctx)
