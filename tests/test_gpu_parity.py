@@ -252,6 +252,31 @@ def test_nucleation_decisions_bit_exact(gpu, oracle):
         lib().pf_ctx_destroy(ctx)
 
 
+def test_nucleation_decisions_fp32_storage(gpu, oracle):
+    """pf_nucleation_f32: decisions on FP32-stored fields equal the oracle's on
+    the same (rounded) values; embeds are the equilibrium compositions rounded
+    to FP32."""
+    from test_oracle import synthetic_nucleation_case
+    fields, a = synthetic_nucleation_case(oracle, nx=70, ny=45, seed=11)
+    fields = [f.astype(np.float32).astype(np.float64) for f in fields]
+    ny, nx = fields[0].shape
+    want, nev, _ = oracle.nucleation(fields, a["sigma_del"], a["sigma_lav"], a["unit_a"],
+                                     a["dx"] * nx, a["dx"] * ny, a["dz"], a["dt"], seed=9,
+                                     call=4)
+    assert nev > 50
+    ctx = ctypes.c_void_p()
+    check(lib().pf_ctx_create(ctypes.byref(ctx)), "ctx")
+    d = [_dev(x, torch.float32) for x in fields]
+    N = _nuc_struct(a, nx, ny, 9, 4)
+    check(lib().pf_nucleation_f32(ctx, nx, ny, *[_p(x) for x in d], ctypes.byref(N), None),
+          "nucleation")
+    torch.cuda.synchronize()
+    for k in range(4):
+        got = d[k].cpu().numpy()
+        assert np.array_equal(got, want[k].astype(np.float32)), k
+    lib().pf_ctx_destroy(ctx)
+
+
 def test_fused_nucleation_matches_separate_pass(gpu, oracle):
     """pf_step with nucleation fused == step, then the standalone pass == the
     oracle's step followed by its nucleation pass."""
