@@ -62,9 +62,9 @@ def step_counts(batch):
 def autotune_batch(batch, launches=8, rows_set=(8, 16, 32, 48, 64), depth_set=(2, 3, 6),
                    launch_steps_set=(1, 2, 3), graph_set=(1, 8)):
     """Measure the current job mix's cell-updates/s for each tile shape
-    (rows, prefetch depth), time steps per launch (one, or two with the
-    temporally blocked march) and launches per captured graph; leave the batch
-    on the fastest.  Rates come from the jobs' step counters, so launches that
+    (rows, prefetch depth), time steps per launch (one, or two or three with
+    the temporally blocked march) and launches per captured graph; leave the
+    batch on the fastest.  Rates come from the jobs' step counters, so launches that
     advance some jobs by one step (nucleation steps, last steps) are counted
     exactly.  The jobs advance during tuning (their targets must allow it).
     Returns (best, table), entries (rate, rows, depth, launch_steps, per_graph)."""

@@ -48,7 +48,7 @@ JOBS = [(64, 64, 1), (130, 71, 0), (37, 200, 1), (200, 33, 0), (1, 5, 1), (63, 6
 @pytest.mark.parametrize("storage", [PF_STORAGE_F64, PF_STORAGE_F32])
 def test_batch_equals_plain_alone(gpu, storage, launch_steps):
     """Each job (odd and even targets, tiny and ragged grids) against the
-    plain path run alone, with one and with two steps per launch."""
+    plain path run alone, with one, two and three steps per launch."""
     dtype = torch.float64 if storage == PF_STORAGE_F64 else torch.float32
     b = Batch(storage, pool_bytes=256 << 20)
     b.set_tile(8, 3)
