@@ -110,3 +110,16 @@ def test_requests_poisson():
     for r in reqs:
         assert 128 <= r.n <= 2048 and 500 <= r.steps <= 20000
     assert len({r.seed for r in reqs}) == len(reqs)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU; --gpus N > 1 exits with a message before any
+    GPU work."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stdout + r.stderr
