@@ -118,6 +118,41 @@ class Simulation:
     def set_tile3(self, rows, depth=0):
         check(lib().pf_ctx_set_tile3(self.ctx, rows, depth), "pf_ctx_set_tile3")
 
+    def tune_tile3(self, rows_set=(6, 8, 12, 16, 24, 32, 48, 64, 72), launches=20):
+        """Time the three-step kernel at each chunk height in `rows_set` on
+        scratch copies of the current fields (the simulation does not advance)
+        and keep the fastest.  The default height is a heuristic within a few
+        per cent of the best on the grids measured; this measures the grid at
+        hand.  Results do not depend on the tile.  Returns (rows, {rows: us
+        per launch})."""
+        if self.steps_per_launch != 3:
+            raise ValueError("tune_tile3 needs three steps per launch")
+        fn = self._stepk[3]
+        st = ctypes.c_void_p(self.stream.cuda_stream)
+        a = [x.clone() for x in self.a]
+        b = [torch.empty_like(x) for x in a]
+
+        def run(n):
+            for i in range(n):
+                x, y = (a, b) if i % 2 == 0 else (b, a)
+                check(fn(self.ctx, self.nx, self.ny, self.s.bc, ctypes.byref(self.P),
+                         *[_ptr(t) for t in x], *[_ptr(t) for t in y], st), "pf_step3")
+
+        times = {}
+        for rows in rows_set:
+            self.set_tile3(rows, 3)
+            run(2)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(self.stream)
+            run(launches)
+            e1.record(self.stream)
+            e1.synchronize()
+            times[rows] = e0.elapsed_time(e1) * 1e3 / launches
+        best = min(times, key=times.get)
+        self.set_tile3(best, 3)
+        return best, times
+
     def step(self, n=1):
         st = ctypes.c_void_p(self.stream.cuda_stream)
         plan = iter_launches(self.steps_done, n, self.steps_per_launch,

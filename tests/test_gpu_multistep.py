@@ -204,3 +204,25 @@ def test_step_loops_equal_single_calls(gpu):
         assert getattr(lib(), name)(ctx, -1, items, None) != 0
         assert getattr(lib(), name)(ctx, 2, None, None) != 0
     lib().pf_ctx_destroy(ctx)
+
+
+def test_tune_tile3_leaves_fields_and_results(gpu):
+    """Simulation.tune_tile3 measures on scratch copies: the fields do not move,
+    the chosen height is one of the candidates, and stepping afterwards gives
+    the same bits as an untuned simulation."""
+    s = M.make_setup(300, 200, bc=1)
+    f0 = M.init_matrix_with_particles(300, 200, s.dx, 1, seed=4, radius=2e-9)
+    tuned = Simulation(s, f0)
+    before = tuned.fields()
+    rows, times = tuned.tune_tile3(rows_set=(6, 16, 48), launches=4)
+    assert rows in (6, 16, 48) and set(times) == {6, 16, 48}
+    after = tuned.fields()
+    for k in range(4):
+        assert np.array_equal(before[k], after[k])
+    tuned.step(31)
+    plain = Simulation(s, f0)
+    plain.step(31)
+    for x, y in zip(tuned.fields(), plain.fields()):
+        assert np.array_equal(x, y)
+    tuned.close()
+    plain.close()
