@@ -226,3 +226,43 @@ def test_tune_tile3_leaves_fields_and_results(gpu):
         assert np.array_equal(x, y)
     tuned.close()
     plain.close()
+
+
+def test_concurrent_simulations_on_two_threads(gpu):
+    """Two simulations stepped at the same time from two host threads, each
+    with its own context and stream, give the same bits as each run alone
+    (the library keeps no shared mutable state outside a context)."""
+    import threading
+    cases = [(M.make_setup(150, 90, bc=0), 21), (M.make_setup(97, 130, bc=1), 22)]
+    inputs = [M.init_matrix_with_particles(s.N[0], s.N[1], s.dx, s.bc, seed=seed,
+                                           radius=2e-9) for s, seed in cases]
+    alone = []
+    for (s, _), f in zip(cases, inputs):
+        sim = Simulation(s, f, nucleation=True, seed=5)
+        sim.step(40)
+        alone.append(sim.fields())
+        sim.close()
+    out = [None, None]
+    errors = []
+
+    def worker(i):
+        try:
+            s, _ = cases[i]
+            stream = torch.cuda.Stream()
+            sim = Simulation(s, inputs[i], nucleation=True, seed=5, stream=stream)
+            for _ in range(10):
+                sim.step(4)
+            out[i] = sim.fields()
+            sim.close()
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not errors, errors
+    for i in range(2):
+        for k in range(4):
+            assert np.array_equal(out[i][k], alone[i][k]), (i, k)
