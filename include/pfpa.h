@@ -290,7 +290,10 @@ double pf_host_max_effective_diffusivity(const double* DCr, const double* DNb);
  * out22: q_Cr = dt D_Cr.(xe_g + A_g mu) as the coefficients of (mu0, mu1, 1),
  * the same for q_Nb, then per phase (delta, Laves) the quadratic form
  * 30 dt L dOmega = g0 mu0^2 + g1 mu0 mu1 + g2 mu1^2 + g3 mu0 + g4 mu1, then
- * 2 omega dt L, 2 alpha dt L and kappa dt L per phase. */
+ * 2 omega dt L, 2 alpha dt L and kappa dt L per phase.  For a symmetric
+ * five-point mask (zero corners, mask[1] = mask[7] != 0, mask[3] = mask[5])
+ * the q coefficients and kappa dt L carry the factor mask[1], by which the
+ * kernels' copy of the mask is divided; for any other mask the factor is 1. */
 int pf_host_step_coefficients(const pf_params* prm, double* out22);
 
 #ifdef __cplusplus

@@ -86,7 +86,14 @@ StepParams make_step(double dt, const double* DCr, const double* DNb, const doub
     P.c[i] = 2.0 * alpha * dtL;
     P.k[i] = kappa[i] * dtL;
   }
-  for (int i = 0; i < 9; i++) P.m[i] = mask[i];
+  // fold the mask's common factor into what the Laplacians multiply
+  const double scale = symmetric_five_point(mask) ? mask[1] : 1.0;
+  for (int i = 0; i < 3; i++) {
+    P.qCr[i] *= scale;
+    P.qNb[i] *= scale;
+  }
+  for (int i = 0; i < 2; i++) P.k[i] *= scale;
+  for (int i = 0; i < 9; i++) P.m[i] = mask[i] / scale;
   return P;
 }
 

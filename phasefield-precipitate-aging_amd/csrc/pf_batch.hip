@@ -500,8 +500,10 @@ int pf_batch_add(pf_batch* b, const pf_job_desc* d, const void* xCr, const void*
     return bfail(PF_ERR_ARG, "bad grid size");
   if (d->bc != PF_BC_NOFLUX && d->bc != PF_BC_PERIODIC) return bfail(PF_ERR_ARG, "bad boundary mode");
   const double* m = d->prm.mask;
-  if (m[0] != 0.0 || m[2] != 0.0 || m[6] != 0.0 || m[8] != 0.0)
-    return bfail(PF_ERR_ARG, "batched form supports the five-point Laplacian only");
+  if (!pf::symmetric_five_point(m))
+    return bfail(PF_ERR_ARG,
+                 "batched form supports the symmetric five-point Laplacian only (zero corners, "
+                 "m[0][1] = m[2][1] != 0, m[1][0] = m[1][2])");
   if (d->nucleation) {
     const double R_scan = 1.75e-9 / (d->dx * (d->Nx + 2));
     if (!(R_scan < 1.0))

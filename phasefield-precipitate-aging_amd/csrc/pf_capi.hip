@@ -93,9 +93,10 @@ int stepk_t(int k, pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, co
   if ((const void*)iCr == (const void*)oCr || (const void*)iNb == (const void*)oNb ||
       (const void*)ipd == (const void*)opd || (const void*)ipl == (const void*)opl)
     return fail(PF_ERR_ARG, "step cannot run in place: old and new buffers must differ");
-  const double* mk = prm->mask;
-  if (mk[0] != 0.0 || mk[2] != 0.0 || mk[6] != 0.0 || mk[8] != 0.0)
-    return fail(PF_ERR_ARG, "multi-step kernels take five-point masks only");
+  if (!pf::symmetric_five_point(prm->mask))
+    return fail(PF_ERR_ARG,
+                "multi-step kernels take symmetric five-point masks only (zero corners, "
+                "m[0][1] = m[2][1] != 0, m[1][0] = m[1][2])");
   const pf::StepParams P = pf::make_step(prm->dt, prm->DCr, prm->DNb, prm->kappa, prm->omega,
                                          prm->Lmob, prm->alpha, prm->mask);
   hipError_t e = pf::launch_stepk<T>(k, nx, ny, bc, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl,

@@ -11,6 +11,7 @@
 #include "data.cuh"
 #include "discretization.cuh"
 #include "mesh.h"
+#include "pf_internal.h"  // symmetric_five_point: the masks the multi-step entry points take
 #include "pfpa.h"
 
 // ---------------------------------------------------------------- errors
@@ -253,8 +254,7 @@ void device_evolution(struct CudaData* dev, const int nx, const int ny, const in
                             dev->phi_lav_old};
   fp_t* const cur_new[4] = {dev->conc_Cr_new, dev->conc_Nb_new, dev->phi_del_new,
                             dev->phi_lav_new};
-  const bool five_point = dev->mask[0] == 0.0 && dev->mask[2] == 0.0 && dev->mask[6] == 0.0 &&
-                          dev->mask[8] == 0.0;
+  const bool five_point = pf::symmetric_five_point(dev->mask);
   if (dev->group != 3 || !five_point || nx < 5 || ny < 5) {
     device_flush(dev);
     one_step(dev, nx, ny, alpha, dt, cur_old, cur_new);

@@ -20,14 +20,32 @@ namespace pf {
 //   phi' = phi + u (G(mu) u + w (2 phi - 1)) - c phi phi_o^2 + k lap(phi)
 // with u = phi (1 - phi), p'(phi) = 30 u^2, phi (phi - 1)(2 phi - 1) = -u (2 phi - 1)
 // and G = 30 dt L dOmega = g[0] mu0^2 + g[1] mu0 mu1 + g[2] mu1^2 + g[3] mu0 + g[4] mu1.
+//
+// A symmetric five-point mask (zero corners, m[1] = m[7] != 0, m[3] = m[5];
+// every five-point mask the reference builds) is stored divided by m[1], and
+// q and k carry the factor m[1] instead, so that a Laplacian is
+// (S + N) + m[3] (W + E) + m[4] C: four operations instead of five.  Any
+// other mask is stored as given (scale 1) and runs the general nine-term sum.
 struct StepParams {
-  double qCr[3], qNb[3];  // dt * (D row) . (A_gamma mu + xe_gamma)
+  double qCr[3], qNb[3];  // scale * dt * (D row) . (A_gamma mu + xe_gamma)
   double g[2][5];         // 30 dt L * grand-potential difference  (delta, Laves)
   double w[2];            // 2 omega dt L
   double c[2];            // 2 alpha dt L
-  double k[2];            // kappa dt L
-  double m[9];            // 3x3 Laplacian mask, row-major (y-1, y, y+1)
+  double k[2];            // scale * kappa dt L
+  double m[9];            // 3x3 Laplacian mask / scale, row-major (y-1, y, y+1)
 };
+
+// The masks the folded five-point Laplacian takes (as given by the caller).
+inline bool symmetric_five_point(const double* m) {
+  return m[0] == 0.0 && m[2] == 0.0 && m[6] == 0.0 && m[8] == 0.0 && m[1] == m[7] &&
+         m[3] == m[5] && m[1] != 0.0 && m[1] - m[1] == 0.0;  // last: finite
+}
+// ... and such a mask once folded (StepParams::m): the kernels pick the
+// five-point arithmetic exactly when this holds.
+inline bool folded_five_point(const double* m) {
+  return m[0] == 0.0 && m[2] == 0.0 && m[6] == 0.0 && m[8] == 0.0 && m[1] == 1.0 &&
+         m[7] == 1.0 && m[3] == m[5];
+}
 
 // Nucleation pass parameters, derived on the host from the reference's
 // device_nucleation arguments (src/discretization.cu:1179-1209).
@@ -88,7 +106,8 @@ hipError_t launch_step(int nx, int ny, int bc, const StepParams& P, const T* iCr
                        const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd, T* opl,
                        const NucParams* nuc, const Scratch& S, const TileConfig& tc,
                        hipStream_t st);
-// k = 2 or 3 steps per launch (five-point mask, no nucleation); grids need Nx, Ny >= 3
+// k = 2 or 3 steps per launch (folded symmetric five-point mask, no nucleation); grids need
+// Nx, Ny >= 3
 template <typename T>
 hipError_t launch_stepk(int k, int nx, int ny, int bc, const StepParams& P, const T* iCr,
                         const T* iNb, const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd,

@@ -213,8 +213,12 @@ __device__ __forceinline__ void nucleate_cell(const NucParams& N, uint32_t cell,
   }
 }
 
-// Laplacians of the four stencil quantities in the reference's mask order
+// Laplacians of the four stencil quantities with the reference's mask
 // (src/discretization.cu:589-595): rows y-1, y, y+1; columns x-1, x, x+1.
+// NINE: the general nine-term sum in mask order.  Otherwise the mask is a
+// folded symmetric five-point one (m[1] = m[7] = 1, m[3] = m[5]; the factor
+// the caller's mask had at m[1] is in q and k), and a Laplacian is
+// (S + N) + m[3] (W + E) + m[4] C.
 template <bool NINE>
 __device__ __forceinline__ void laplacians(const double* m, const Point& S, const Point& W,
                                            const Point& C, const Point& E, const Point& Nn,
@@ -243,11 +247,8 @@ __device__ __forceinline__ void laplacians(const double* m, const Point& S, cons
       v = fma(m[7], n[f], v);
       v = fma(m[8], ne[f], v);
     } else {
-      v = m[1] * s[f];
-      v = fma(m[3], w[f], v);
+      v = fma(m[3], w[f] + e[f], s[f] + n[f]);
       v = fma(m[4], c[f], v);
-      v = fma(m[5], e[f], v);
-      v = fma(m[7], n[f], v);
     }
     lap[f] = v;
   }
@@ -444,7 +445,7 @@ __device__ __forceinline__ void march(
 // Nx-1).  No-flux images are copies of the adjacent interior cell's value at
 // the same level: taken from the neighbouring lane in x, and from the adjacent
 // row of the level's window in y.  So K steps here give the same result as K
-// one-step launches.  Five-point masks, no nucleation (the caller runs
+// one-step launches.  Symmetric five-point masks, no nucleation (the caller runs
 // nucleation steps one at a time).
 //
 // IMG: whether this wave's strip and chunk can meet a no-flux image (an edge

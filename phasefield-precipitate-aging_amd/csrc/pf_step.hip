@@ -77,7 +77,7 @@ hipError_t launch_step(int nx, int ny, int bc, const StepParams& P, const T* iCr
                        const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd, T* opl,
                        const NucParams* nuc, const Scratch& S, const TileConfig& tc,
                        hipStream_t st) {
-  const bool nine = P.m[0] != 0.0 || P.m[2] != 0.0 || P.m[6] != 0.0 || P.m[8] != 0.0;
+  const bool nine = !folded_five_point(P.m);  // the general nine-term Laplacian
   NucParams N{};
   if (nuc) N = *nuc;
 #define PF_LAUNCH(BCV, NI, NU)                                                                  \

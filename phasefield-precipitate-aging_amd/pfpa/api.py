@@ -255,8 +255,7 @@ def device_evolution(dev, nx, ny, nm, bx, by, alpha, dt):
     three to a launch (see the module docstring)."""
     dev.pending.clear()
     old, new = _fields(dev, "_old"), _fields(dev, "_new")
-    m = dev.params.mask
-    five_point = m[0] == 0.0 and m[2] == 0.0 and m[6] == 0.0 and m[8] == 0.0
+    five_point = M.symmetric_five_point(dev.params.mask)
     if dev.group != 3 or not five_point or nx < 5 or ny < 5:
         device_flush(dev)
         _launch(dev, lib().pf_step_f64, nx, ny, alpha, dt, old, new, None)

@@ -76,6 +76,16 @@ def nine_point_mask(dx, dy):
             c, 4.0 / (6.0 * dy * dy), c]
 
 
+def symmetric_five_point(mask):
+    """Whether a row-major 3x3 mask is a symmetric five-point one (zero corners,
+    m[0][1] = m[2][1] != 0 and finite, m[1][0] = m[1][2]): the masks the two- and
+    three-step kernels and the batched form take (every five-point mask the
+    reference builds, src/mesh.cpp:187-196, is one)."""
+    m = [float(v) for v in mask]
+    return (m[0] == 0.0 and m[2] == 0.0 and m[6] == 0.0 and m[8] == 0.0 and m[1] == m[7]
+            and m[3] == m[5] and m[1] != 0.0 and math.isfinite(m[1]))
+
+
 def max_effective_diffusivity(DCr, DNb):
     a = (ctypes.c_double * 2)(*DCr)
     b = (ctypes.c_double * 2)(*DNb)

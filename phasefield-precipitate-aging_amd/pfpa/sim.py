@@ -9,6 +9,7 @@ import numpy as np
 import torch
 
 from ._lib import PfNucParams, PfParams, check, lib
+from . import model as M
 from .model import LATTICE_CONST, IFCE_WIDTH, SIGMA
 
 
@@ -103,8 +104,7 @@ class Simulation:
             steps_per_launch = 3
         if steps_per_launch not in (1, 2, 3):
             raise ValueError("steps_per_launch must be 1, 2 or 3")
-        five_point = all(setup.mask[k] == 0.0 for k in (0, 2, 6, 8))
-        if not (five_point and self.nx >= 5 and self.ny >= 5):
+        if not (M.symmetric_five_point(setup.mask) and self.nx >= 5 and self.ny >= 5):
             steps_per_launch = 1
         self.steps_per_launch = steps_per_launch
         self._summ = lib().pf_summarize_f64 if dtype == torch.float64 else lib().pf_summarize_f32

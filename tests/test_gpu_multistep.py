@@ -147,6 +147,13 @@ def test_multistep_errors(ctx, k):
     P9 = _params(s, m9)
     assert two(ctx, 10, 10, 0, ctypes.byref(P9), *map(_p, f[:4]), *map(_p, f[4:]), None) != 0
     assert b"five-point" in lib().pf_last_error()
+    # five-point but not symmetric (west and east weights differ)
+    m5 = list(s.mask)
+    m5[3] *= 1.25
+    m5[4] -= 0.25 * s.mask[3]
+    P5 = _params(s, m5)
+    assert two(ctx, 10, 10, 0, ctypes.byref(P5), *map(_p, f[:4]), *map(_p, f[4:]), None) != 0
+    assert b"symmetric five-point" in lib().pf_last_error()
     # interior smaller than 3x3
     assert two(ctx, 4, 10, 0, ctypes.byref(P), *map(_p, f[:4]), *map(_p, f[4:]), None) != 0
     # bad boundary mode, bad tile

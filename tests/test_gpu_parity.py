@@ -464,26 +464,35 @@ def _masks(dx, dy):
     return five, nine
 
 
-@pytest.mark.parametrize("nine", [False, True])
+@pytest.mark.parametrize("kind", ["five", "nine", "lopsided"])
 @pytest.mark.parametrize("bc", [0, 1])
-def test_anisotropic_masks(gpu, oracle, bc, nine):
+def test_anisotropic_masks(gpu, oracle, bc, kind):
     """Masks for dy = 1.3 dx.  The reference's nine-point mask then does not
     sum to zero (its entries sum to -2 (dx - dy)^2 / (6 dx^2 dy^2)), so the
     mask sum of a uniform field is not zero and the constant part of
     q = dt D x_gamma contributes to every update; the five-point mask sums to
-    zero.  One step per launch and, for the five-point mask, three."""
+    zero.  One step per launch and, for the five-point mask, three.
+    "lopsided": a five-point mask with unequal west and east weights, which the
+    folded symmetric Laplacian cannot take: it runs the general nine-term sum,
+    one step per launch, whatever steps per launch are asked for."""
     Nx, Ny = 96, 70
     dx = 0.25e-9
     dy = 1.3 * dx
     s = M.make_setup(Nx, Ny, bc=bc, dt_rule=5e-8)
     five, ninem = _masks(dx, dy)
-    s.mask = ninem if nine else five
-    if nine:
+    if kind == "lopsided":
+        five = list(five)
+        five[3] *= 1.2
+        five[5] *= 0.8
+    s.mask = ninem if kind == "nine" else five
+    assert M.symmetric_five_point(s.mask) == (kind == "five")
+    if kind == "nine":
         assert abs(sum(ninem)) > 1e-3 * max(abs(v) for v in ninem)
     f0 = M.init_matrix_with_particles(Nx, Ny, dx, bc, seed=8, radius=2e-9)
     want = oracle.run(f0, oracle_params(oracle, s), bc, 25)
-    for spl in ((1,) if nine else (1, 3)):
+    for spl in ((1,) if kind == "nine" else (1, 3)):
         sim = Simulation(s, f0, steps_per_launch=spl)
+        assert sim.steps_per_launch == (spl if kind == "five" else 1)
         sim.step(25)
         got = sim.fields()
         sim.close()

@@ -188,6 +188,11 @@ def test_step_coefficients_against_oracle_paraboloids(oracle):
     out = (ctypes.c_double * 22)()
     assert pfpa.lib().pf_host_step_coefficients(ctypes.byref(P), out) == 0
     c = np.array(out[:])
+    # the five-point mask's factor 1/dy^2 rides on q and kappa dt L
+    assert M.symmetric_five_point(s.mask)
+    scale = s.mask[1]
+    c[0:6] /= scale
+    c[20:22] /= scale
     qCr, qNb, G = c[0:3], c[3:6], c[6:16].reshape(2, 5)
 
     def A(a):
@@ -224,6 +229,46 @@ def test_step_coefficients_against_oracle_paraboloids(oracle):
         assert c[18 + ph] == pytest.approx(2 * s.alpha * dtL, rel=1e-15)
         assert c[20 + ph] == pytest.approx(s.kappa[ph] * dtL, rel=1e-15)
     assert pfpa.lib().pf_host_step_coefficients(None, out) != 0
+
+
+def test_step_coefficients_mask_fold():
+    """A symmetric five-point mask's factor mask[1] moves onto q and kappa dt L
+    (the kernels divide their copy of the mask by it); a mask the folded
+    Laplacian cannot take (unequal west/east weights, nine points, a zero
+    mask[1]) leaves every coefficient as it is."""
+    from pfpa._lib import PfParams
+    s = M.make_setup(64, 64)
+
+    def coeffs(mask):
+        P = PfParams()
+        P.dt = s.dt
+        for k in range(2):
+            P.DCr[k], P.DNb[k] = s.DCr[k], s.DNb[k]
+            P.kappa[k], P.omega[k], P.Lmob[k] = s.kappa[k], s.omega[k], s.Lmob[k]
+        P.alpha = s.alpha
+        for k in range(9):
+            P.mask[k] = mask[k]
+        out = (ctypes.c_double * 22)()
+        assert pfpa.lib().pf_host_step_coefficients(ctypes.byref(P), out) == 0
+        return np.array(out[:])
+
+    five = list(s.mask)
+    lop = list(five)
+    lop[3] *= 1.2
+    nine = M.nine_point_mask(s.dx, s.dx)
+    x_only = [0.0, 0.0, 0.0, five[3], -2 * five[3], five[3], 0.0, 0.0, 0.0]
+    assert M.symmetric_five_point(five)
+    for m in (lop, nine, x_only):
+        assert not M.symmetric_five_point(m)
+    base = coeffs(lop)
+    for m in (nine, x_only):
+        assert np.array_equal(coeffs(m), base)
+    folded = coeffs(five)
+    scale = five[1]
+    idx = list(range(0, 6)) + [20, 21]
+    assert np.array_equal(folded[idx], base[idx] * scale)
+    rest = [i for i in range(22) if i not in idx]
+    assert np.array_equal(folded[rest], base[rest])
 
 
 @pytest.mark.parametrize("code", [53, 93, 0])
