@@ -36,6 +36,28 @@ int check_ptrs(const void* a, const void* b, const void* c, const void* d) {
   return PF_OK;
 }
 
+// A step reads the four old fields while it writes the four new ones, so no
+// new field may overlap an old field or another new field (each array holds
+// nx * ny cells).
+template <typename T>
+int check_disjoint(int nx, int ny, const T* i0, const T* i1, const T* i2, const T* i3,
+                   const T* o0, const T* o1, const T* o2, const T* o3) {
+  const uintptr_t n = (uintptr_t)nx * (uintptr_t)ny * sizeof(T);
+  const uintptr_t in[4] = {(uintptr_t)i0, (uintptr_t)i1, (uintptr_t)i2, (uintptr_t)i3};
+  const uintptr_t out[4] = {(uintptr_t)o0, (uintptr_t)o1, (uintptr_t)o2, (uintptr_t)o3};
+  auto overlap = [n](uintptr_t a, uintptr_t b) { return a < b + n && b < a + n; };
+  for (int i = 0; i < 4; i++) {
+    for (int j = 0; j < 4; j++)
+      if (overlap(out[i], in[j]))
+        return fail(PF_ERR_ARG,
+                    "step cannot run in place: the new field buffers must not overlap the old ones");
+    for (int j = 0; j < i; j++)
+      if (overlap(out[i], out[j]))
+        return fail(PF_ERR_ARG, "the four new field buffers must not overlap each other");
+  }
+  return PF_OK;
+}
+
 template <typename T>
 int step_t(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const T* iCr,
            const T* iNb, const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd, T* opl,
@@ -45,9 +67,7 @@ int step_t(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const T* i
   if (int r = check_ptrs(iCr, iNb, ipd, ipl)) return r;
   if (int r = check_ptrs(oCr, oNb, opd, opl)) return r;
   if (bc != PF_BC_NOFLUX && bc != PF_BC_PERIODIC) return fail(PF_ERR_ARG, "unknown boundary mode");
-  if ((const void*)iCr == (const void*)oCr || (const void*)iNb == (const void*)oNb ||
-      (const void*)ipd == (const void*)opd || (const void*)ipl == (const void*)opl)
-    return fail(PF_ERR_ARG, "step cannot run in place: old and new buffers must differ");
+  if (int r = check_disjoint<T>(nx, ny, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl)) return r;
   const pf::StepParams P = pf::make_step(prm->dt, prm->DCr, prm->DNb, prm->kappa, prm->omega,
                                          prm->Lmob, prm->alpha, prm->mask);
   hipStream_t st = (hipStream_t)stream;
@@ -90,9 +110,7 @@ int stepk_t(int k, pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, co
   if (int r = check_ptrs(iCr, iNb, ipd, ipl)) return r;
   if (int r = check_ptrs(oCr, oNb, opd, opl)) return r;
   if (bc != PF_BC_NOFLUX && bc != PF_BC_PERIODIC) return fail(PF_ERR_ARG, "unknown boundary mode");
-  if ((const void*)iCr == (const void*)oCr || (const void*)iNb == (const void*)oNb ||
-      (const void*)ipd == (const void*)opd || (const void*)ipl == (const void*)opl)
-    return fail(PF_ERR_ARG, "step cannot run in place: old and new buffers must differ");
+  if (int r = check_disjoint<T>(nx, ny, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl)) return r;
   if (!pf::symmetric_five_point(prm->mask))
     return fail(PF_ERR_ARG,
                 "multi-step kernels take symmetric five-point masks only (zero corners, "

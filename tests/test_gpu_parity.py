@@ -422,6 +422,23 @@ def test_errors_are_reported(gpu):
     assert rc == 1 and b"in place" in L.pf_last_error()
     rc = L.pf_step_f64(ctx, 4, 4, 7, ctypes.byref(P), *[_p(t)] * 8, None, None)
     assert rc == 1
+    # a new field on an old field of another name, two new fields on one buffer, or a
+    # new field overlapping an old one by part of its length
+    old = [torch.zeros(16, dtype=torch.float64, device="cuda") for _ in range(4)]
+    new = [torch.zeros(16, dtype=torch.float64, device="cuda") for _ in range(4)]
+    big = torch.zeros(64, dtype=torch.float64, device="cuda")
+    for outs, what in (([new[0], old[2], new[2], new[3]], b"in place"),
+                       ([new[0], new[1], new[1], new[3]], b"each other"),
+                       ([big[8:24], big[24:40], big[40:56], new[3]], None)):
+        for ins in (old, [big[0:16]] + old[1:]):
+            rc = L.pf_step_f64(ctx, 4, 4, 0, ctypes.byref(P), *map(_p, ins), *map(_p, outs),
+                               None, None)
+            if what is None and ins is old:
+                assert rc == 0, L.pf_last_error()   # disjoint views of one buffer are fine
+            else:
+                assert rc == 1
+                assert (what or b"in place") in L.pf_last_error()
+    torch.cuda.synchronize()
     with pytest.raises(pfpa.PfError):
         check(L.pf_thermo_eval(9, 1, _p(t), _p(t), None), "eval")
     L.pf_ctx_destroy(ctx)
