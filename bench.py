@@ -392,7 +392,8 @@ def serving_section(args):
 
 
 def precision_section(args):
-    """FP32 storage vs FP64 on 8192^2 (same init as the plain case): max field
+    """FP32 storage, without and with a composition offset (the median
+    composition), vs FP64 on 8192^2 (same init as the plain case): max field
     differences, phase fractions and mass drift at 1k/10k/20k steps."""
     import torch
 
@@ -403,11 +404,13 @@ def precision_section(args):
     f0 = M.init_enriched_band(N, N, 1, seed=42)
     s64 = Simulation(s, f0, dtype=torch.float64, nucleation=True, seed=42)
     s32 = Simulation(s, f0, dtype=torch.float32, nucleation=True, seed=42)
+    s32o = Simulation(s, f0, dtype=torch.float32, nucleation=True, seed=42,
+                      storage_offset="median")
     del f0
     m0 = s64.summarize()
     rows = []
     done = 0
-    secs = {64: 0.0, 32: 0.0}   # GPU time of each form's stepping (HIP events)
+    secs = {64: 0.0, 32: 0.0, "32o": 0.0}   # GPU time of each form's stepping (HIP events)
 
     def timed_steps(sim, n, key):
         e0 = torch.cuda.Event(enable_timing=True)
@@ -421,20 +424,32 @@ def precision_section(args):
     for mark in args.precision_marks:
         timed_steps(s64, mark - done, 64)
         timed_steps(s32, mark - done, 32)
+        timed_steps(s32o, mark - done, "32o")
         done = mark
-        dCr = float((s64.a[0] - s32.a[0].double()).abs().max())
-        dNb = float((s64.a[1] - s32.a[1].double()).abs().max())
-        r64 = s64.summarize()
-        r32 = s32.summarize()
-        rows.append({"step": mark, "max_abs_dx_Cr": dCr, "max_abs_dx_Nb": dNb,
+
+        def max_dx(sim, k):
+            return float((s64.a[k] - (sim.a[k].double() + sim.xoff[k])).abs().max())
+
+        r64, r32, r32o = s64.summarize(), s32.summarize(), s32o.summarize()
+        rows.append({"step": mark, "max_abs_dx_Cr": max_dx(s32, 0),
+                     "max_abs_dx_Nb": max_dx(s32, 1),
+                     "max_abs_dx_Cr_offset": max_dx(s32o, 0),
+                     "max_abs_dx_Nb_offset": max_dx(s32o, 1),
                      "delta_frac_64": r64[3], "delta_frac_32": r32[3],
+                     "delta_frac_32_offset": r32o[3],
                      "laves_frac_64": r64[4], "laves_frac_32": r32[4],
+                     "laves_frac_32_offset": r32o[4],
                      "mass_drift_Cr_64": r64[0] - m0[0], "mass_drift_Cr_32": r32[0] - m0[0],
-                     "mass_drift_Nb_64": r64[1] - m0[1], "mass_drift_Nb_32": r32[1] - m0[1]})
+                     "mass_drift_Cr_32_offset": r32o[0] - m0[0],
+                     "mass_drift_Nb_64": r64[1] - m0[1], "mass_drift_Nb_32": r32[1] - m0[1],
+                     "mass_drift_Nb_32_offset": r32o[1] - m0[1]})
         log("precision", json.dumps(rows[-1]))
+    offset = list(s32o.xoff)
     s64.close()
     s32.close()
+    s32o.close()
     rates = {"fp64": N * N * done / secs[64], "fp32_storage": N * N * done / secs[32],
+             "fp32_storage_offset": N * N * done / secs["32o"], "offset": offset,
              "unit": "cell-updates/s", "steps": done}
     log("precision rates", json.dumps(rates))
     return rows, rates

@@ -71,6 +71,14 @@ int pf_device_count(void);
  * XCD-aware block order on/off, rows of loads kept in flight (2, 3 or 6;
  * 0 = default 3).  Results do not depend on the tile. */
 int pf_ctx_set_tile(pf_ctx* ctx, int rows, int xcd_remap, int depth);
+/* Composition offset of FP32 storage (no counterpart in the reference, whose
+ * fields are FP64): on this context the conc_Cr / conc_Nb arrays of every
+ * *_f32 entry point (step, two/three-step, nucleation, summarize, energy) hold
+ * x - x_cr and x - x_nb.  With the offsets near the mean composition the FP32
+ * values are ~10x smaller, so each step's small composition increment keeps
+ * 3-4 more bits.  Default 0 (arrays hold x); FP64 entry points ignore it.
+ * Returns PF_ERR_ARG for a non-finite offset. */
+int pf_ctx_set_storage_offset(pf_ctx* ctx, double x_cr, double x_nb);
 
 /* One fused explicit-Euler step, old -> new.  Replaces the per-step sequence
  * device_boundaries, device_fictitious, fictitious_boundaries,
@@ -97,7 +105,7 @@ int pf_step_f32(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm,
  * memory (temporal blocking).  The result equals two pf_step_* calls without
  * nucleation, the first writing a scratch level that the second reads: the
  * reference's step sequence (src/main.cpp:289-305, 318-321) run twice.
- * Five-point masks only; the interior must be at least 3x3.  Nucleation steps
+ * Symmetric five-point masks only; the interior must be at least 3x3.  Nucleation steps
  * (src/main.cpp:307-316) go through pf_step_* one at a time. */
 int pf_step2_f64(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm,
                  const double* conc_Cr_old, const double* conc_Nb_old,
@@ -217,7 +225,7 @@ typedef struct pf_job_desc {
 	int bc;                /* PF_BC_NOFLUX or PF_BC_PERIODIC */
 	int nucleation;        /* 1 = nucleation pass every nuc_every steps (step % nuc_every == 0) */
 	double dx;             /* mesh spacing (m), used by the nucleation pass */
-	pf_params prm;         /* dt, diffusivity, kappa, omega, mobility, alpha, five-point mask */
+	pf_params prm;         /* dt, diffusivity, kappa, omega, mobility, alpha, symmetric five-point mask */
 	uint64_t steps;        /* steps to run */
 	uint64_t seed;         /* Philox key */
 	uint32_t instance;     /* folded into the random counter: unique per simulation */

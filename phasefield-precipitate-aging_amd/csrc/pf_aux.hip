@@ -60,8 +60,8 @@ NucParams make_nuc(double sigma_del, double sigma_lav, double unit_a, double dx,
 }
 
 StepParams make_step(double dt, const double* DCr, const double* DNb, const double* kappa,
-                     const double* omega, const double* Lmob, double alpha,
-                     const double* mask) {
+                     const double* omega, const double* Lmob, double alpha, const double* mask,
+                     const double* xoff) {
   StepParams P{};
   // q = dt D (xe_g + A_g mu): coefficients of mu0, mu1 and the constant
   const double* rows[2] = {DCr, DNb};
@@ -94,6 +94,10 @@ StepParams make_step(double dt, const double* DCr, const double* DNb, const doub
   }
   for (int i = 0; i < 2; i++) P.k[i] *= scale;
   for (int i = 0; i < 9; i++) P.m[i] = mask[i] / scale;
+  for (int c = 0; c < 2; c++) {
+    P.xoff[c] = xoff ? xoff[c] : 0.0;
+    P.xe_g[c] = XE[0][c] - P.xoff[c];
+  }
   return P;
 }
 
@@ -124,7 +128,7 @@ __global__ __launch_bounds__(256) void nucleation_kernel(T* __restrict__ xCr, T*
   }
   if (!(phi_pre < 1e-10)) return;
   const uint32_t idx = (uint32_t)(nx * y + x);
-  const double cCr = (double)xCr[idx], cNb = (double)xNb[idx];
+  const double cCr = (double)xCr[idx] + N.xoff[0], cNb = (double)xNb[idx] + N.xoff[1];
   double u1, u2;
   nuc_uniforms(N.seed, N.call, N.instance, idx, u1, u2);
   const double dG = g(0, cCr, cNb);
@@ -139,8 +143,8 @@ __global__ __launch_bounds__(256) void nucleation_kernel(T* __restrict__ xCr, T*
     if (!(Pn - u > 0)) continue;
     const double Rp = Rstar / N.dx_arg;
     if (in_place && Rp < 1.0) {
-      xCr[idx] = (T)XE[1 + ph][0];
-      xNb[idx] = (T)XE[1 + ph][1];
+      xCr[idx] = (T)(XE[1 + ph][0] - N.xoff[0]);
+      xNb[idx] = (T)(XE[1 + ph][1] - N.xoff[1]);
       if (ph == 0)
         pd[idx] = (T)1.0;
       else
@@ -159,8 +163,10 @@ __global__ __launch_bounds__(256) void nucleation_kernel(T* __restrict__ xCr, T*
 // Deferred embeds, one thread, deterministic order: centre-only events first,
 // then wide ones, each by (cell, phase).  embed_OPC_kernel semantics
 // (src/discretization.cu:1046-1077).  Clears the event count afterwards.
+// Events carry absolute compositions; the arrays hold x - (off0, off1).
 template <typename T>
-__global__ void apply_events_kernel(T* xCr, T* xNb, T* pd, T* pl, int nx, int ny, Scratch S) {
+__global__ void apply_events_kernel(T* xCr, T* xNb, T* pd, T* pl, int nx, int ny, Scratch S,
+                                    double off0, double off1) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   uint32_t n = S.counters[0];
   if (n > S.event_capacity) n = S.event_capacity;
@@ -195,12 +201,12 @@ __global__ void apply_events_kernel(T* xCr, T* xNb, T* pd, T* pl, int nx, int ny
         const double r = sqrt((double)(i * i + j * j));
         if (idn >= 0 && idn < ncell) {
           if (r <= R) {
-            xCr[idn] = (T)xeCr;
-            xNb[idn] = (T)xeNb;
+            xCr[idn] = (T)(xeCr - off0);
+            xNb[idn] = (T)(xeNb - off1);
             phi[idn] = (T)1.0;
           } else {
-            if (r <= RdCr) xCr[idn] = (T)XE[0][0];
-            if (r <= RdNb) xNb[idn] = (T)XE[0][1];
+            if (r <= RdCr) xCr[idn] = (T)(XE[0][0] - off0);
+            if (r <= RdNb) xNb[idn] = (T)(XE[0][1] - off1);
           }
         }
       }
@@ -218,20 +224,20 @@ hipError_t launch_nucleation(int nx, int ny, T* xCr, T* xNb, T* pd, T* pl, const
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(apply_events_kernel<T>, dim3(1), dim3(64), 0, st, xCr, xNb, pd, pl, nx, ny,
-                     S);
+                     S, N.xoff[0], N.xoff[1]);
   return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_apply_events(int nx, int ny, T* xCr, T* xNb, T* pd, T* pl, const Scratch& S,
-                               hipStream_t st) {
+                               const double* xoff, hipStream_t st) {
   hipLaunchKernelGGL(apply_events_kernel<T>, dim3(1), dim3(64), 0, st, xCr, xNb, pd, pl, nx, ny,
-                     S);
+                     S, xoff ? xoff[0] : 0.0, xoff ? xoff[1] : 0.0);
   return hipGetLastError();
 }
 template hipError_t launch_apply_events<double>(int, int, double*, double*, double*, double*,
-                                                const Scratch&, hipStream_t);
+                                                const Scratch&, const double*, hipStream_t);
 template hipError_t launch_apply_events<float>(int, int, float*, float*, float*, float*,
-                                               const Scratch&, hipStream_t);
+                                               const Scratch&, const double*, hipStream_t);
 
 template hipError_t launch_nucleation<double>(int, int, double*, double*, double*, double*,
                                               const NucParams&, const Scratch&, hipStream_t);
@@ -293,9 +299,11 @@ __global__ __launch_bounds__(256) void summarize_partial_kernel(const T* __restr
 
 // Final pass over the per-block partials of `nq` quantities (stride 2*nq
 // doubles per block): block k merges quantity k, 64 threads over a fixed
-// partition, then a fixed tree.  out[k] = sum * scale.
+// partition, then a fixed tree.  out[k] = sum * scale (+ add0 / add1 for
+// quantities 0 / 1).
 __global__ __launch_bounds__(64) void reduce_final_kernel(const double* partials, int nblocks,
-                                                          int nq, double scale, double* out) {
+                                                          int nq, double scale, double* out,
+                                                          double add0, double add1) {
   const int k = blockIdx.x, t = threadIdx.x;
   Kahan a{0.0, 0.0};
   for (int b = t; b < nblocks; b += 64) {
@@ -309,12 +317,14 @@ __global__ __launch_bounds__(64) void reduce_final_kernel(const double* partials
     if (t < w) kmerge(sh[t], sh[t + w]);
     __syncthreads();
   }
-  if (t == 0) out[k] = (sh[0].s + sh[0].c) * scale;
+  if (t == 0) out[k] = (sh[0].s + sh[0].c) * scale + (k == 0 ? add0 : (k == 1 ? add1 : 0.0));
 }
 
+// The mean compositions are the offset plus the mean of the stored values.
 template <typename T>
 hipError_t launch_summarize(int nx, int ny, const T* xCr, const T* xNb, const T* pd,
-                            const T* pl, double* out5, const Scratch& S, hipStream_t st) {
+                            const T* pl, const double* xoff, double* out5, const Scratch& S,
+                            hipStream_t st) {
   const long N = (long)(nx - 2) * (ny - 2);
   int blocks = (int)((N + 255) / 256);
   if (blocks > REDUCE_BLOCKS) blocks = REDUCE_BLOCKS;
@@ -324,14 +334,15 @@ hipError_t launch_summarize(int nx, int ny, const T* xCr, const T* xNb, const T*
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(reduce_final_kernel, dim3(5), dim3(64), 0, st, S.partials, blocks, 5,
-                     1.0 / (double)N, out5);
+                     1.0 / (double)N, out5, xoff ? xoff[0] : 0.0, xoff ? xoff[1] : 0.0);
   return hipGetLastError();
 }
 template hipError_t launch_summarize<double>(int, int, const double*, const double*,
-                                             const double*, const double*, double*,
-                                             const Scratch&, hipStream_t);
+                                             const double*, const double*, const double*,
+                                             double*, const Scratch&, hipStream_t);
 template hipError_t launch_summarize<float>(int, int, const float*, const float*, const float*,
-                                            const float*, double*, const Scratch&, hipStream_t);
+                                            const float*, const double*, double*,
+                                            const Scratch&, hipStream_t);
 
 // ------------------------------------------------------------ free energy
 // src/alloy625.cpp:752-784 (gibbs), :786-806 (maskedgradient) and :865-919
@@ -354,7 +365,7 @@ __global__ __launch_bounds__(256) void energy_partial_kernel(
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < N; i += (long)gridDim.x * 256) {
     const int y = (int)(i / Nx), x = (int)(i - (long)y * Nx);
     const size_t o = (size_t)(y + 1) * nx + (x + 1);
-    const double cr = (double)xCr[o], nb = (double)xNb[o];
+    const double cr = (double)xCr[o] + E.xoff[0], nb = (double)xNb[o] + E.xoff[1];
     const double fd = (double)pd[o], fl = (double)pl[o];
     const double pD = p(fd), pL = p(fl), pG = 1.0 - pD - pL;
     double mu0, mu1;
@@ -425,7 +436,7 @@ hipError_t launch_energy(int nx, int ny, int bc, const EnergyParams& E, const T*
   // the two sums land in out3[1..2] via a scratch slot after the partials
   double* sums = S.partials + (size_t)REDUCE_BLOCKS * 10;
   hipLaunchKernelGGL(reduce_final_kernel, dim3(2), dim3(64), 0, st, S.partials, blocks, 2, 1.0,
-                     sums);
+                     sums, 0.0, 0.0);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(energy_total_kernel, dim3(1), dim3(64), 0, st, sums, out3);
   return hipGetLastError();

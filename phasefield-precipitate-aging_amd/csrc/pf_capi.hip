@@ -11,6 +11,7 @@ struct pf_ctx {
   pf::Scratch scratch;
   pf::TileConfig tile;
   int device;
+  double xoff[2];  // composition offset of FP32-stored arrays (pf_ctx_set_storage_offset)
 };
 
 namespace {
@@ -29,6 +30,13 @@ int check_grid(int nx, int ny) {
   if ((long long)nx * (long long)ny >= (1LL << 31))
     return fail(PF_ERR_ARG, "grid too large for 32-bit cell indices");
   return PF_OK;
+}
+
+// The composition offset the arrays of storage type T hold on this context:
+// FP32 storage may carry one, FP64 storage never does.
+template <typename T>
+const double* storage_offset(const pf_ctx* ctx) {
+  return sizeof(T) == sizeof(float) ? ctx->xoff : nullptr;
 }
 
 int check_ptrs(const void* a, const void* b, const void* c, const void* d) {
@@ -68,21 +76,24 @@ int step_t(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const T* i
   if (int r = check_ptrs(oCr, oNb, opd, opl)) return r;
   if (bc != PF_BC_NOFLUX && bc != PF_BC_PERIODIC) return fail(PF_ERR_ARG, "unknown boundary mode");
   if (int r = check_disjoint<T>(nx, ny, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl)) return r;
+  const double* xoff = storage_offset<T>(ctx);
   const pf::StepParams P = pf::make_step(prm->dt, prm->DCr, prm->DNb, prm->kappa, prm->omega,
-                                         prm->Lmob, prm->alpha, prm->mask);
+                                         prm->Lmob, prm->alpha, prm->mask, xoff);
   hipStream_t st = (hipStream_t)stream;
   hipError_t e;
   if (nuc) {
-    const pf::NucParams N =
+    pf::NucParams N =
         pf::make_nuc(nuc->sigma_del, nuc->sigma_lav, nuc->unit_a, nuc->dx, nuc->dy, nuc->dz,
                      nuc->dt, nuc->seed, nuc->call, nuc->instance, nx, ny);
+    N.xoff[0] = P.xoff[0];
+    N.xoff[1] = P.xoff[1];
     if (N.R_scan < 1.0) {
       // fused: decisions on each cell's new values, centre embeds in place,
       // wider embeds deferred to the event pass
       e = pf::launch_step<T>(nx, ny, bc, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, &N,
                              ctx->scratch, ctx->tile, st);
       if (e != hipSuccess) return hip_fail(e, "step kernel");
-      e = pf::launch_apply_events<T>(nx, ny, oCr, oNb, opd, opl, ctx->scratch, st);
+      e = pf::launch_apply_events<T>(nx, ny, oCr, oNb, opd, opl, ctx->scratch, xoff, st);
       if (e != hipSuccess) return hip_fail(e, "nucleation embed kernel");
       return PF_OK;
     }
@@ -116,7 +127,7 @@ int stepk_t(int k, pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, co
                 "multi-step kernels take symmetric five-point masks only (zero corners, "
                 "m[0][1] = m[2][1] != 0, m[1][0] = m[1][2])");
   const pf::StepParams P = pf::make_step(prm->dt, prm->DCr, prm->DNb, prm->kappa, prm->omega,
-                                         prm->Lmob, prm->alpha, prm->mask);
+                                         prm->Lmob, prm->alpha, prm->mask, storage_offset<T>(ctx));
   hipError_t e = pf::launch_stepk<T>(k, nx, ny, bc, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl,
                                      ctx->tile, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, k == 3 ? "three-step kernel" : "two-step kernel");
@@ -139,9 +150,13 @@ int nucleation_t(pf_ctx* ctx, int nx, int ny, T* xCr, T* xNb, T* pd, T* pl,
   if (!ctx || !nuc) return fail(PF_ERR_ARG, "null context or parameters");
   if (int r = check_grid(nx, ny)) return r;
   if (int r = check_ptrs(xCr, xNb, pd, pl)) return r;
-  const pf::NucParams N =
+  pf::NucParams N =
       pf::make_nuc(nuc->sigma_del, nuc->sigma_lav, nuc->unit_a, nuc->dx, nuc->dy, nuc->dz,
                    nuc->dt, nuc->seed, nuc->call, nuc->instance, nx, ny);
+  if (const double* xoff = storage_offset<T>(ctx)) {
+    N.xoff[0] = xoff[0];
+    N.xoff[1] = xoff[1];
+  }
   hipError_t e =
       pf::launch_nucleation<T>(nx, ny, xCr, xNb, pd, pl, N, ctx->scratch, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "nucleation kernel");
@@ -154,8 +169,8 @@ int summarize_t(pf_ctx* ctx, int nx, int ny, const T* xCr, const T* xNb, const T
   if (!ctx || !out5) return fail(PF_ERR_ARG, "null context or output");
   if (int r = check_grid(nx, ny)) return r;
   if (int r = check_ptrs(xCr, xNb, pd, pl)) return r;
-  hipError_t e = pf::launch_summarize<T>(nx, ny, xCr, xNb, pd, pl, out5, ctx->scratch,
-                                         (hipStream_t)stream);
+  hipError_t e = pf::launch_summarize<T>(nx, ny, xCr, xNb, pd, pl, storage_offset<T>(ctx), out5,
+                                         ctx->scratch, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "summarize kernels");
   return PF_OK;
 }
@@ -178,6 +193,9 @@ int energy_t(pf_ctx* ctx, int nx, int ny, int bc, double dx, double dy, const do
     E.omega[k] = omega[k];
   }
   E.Omega = Omega;
+  const double* xoff = storage_offset<T>(ctx);
+  E.xoff[0] = xoff ? xoff[0] : 0.0;
+  E.xoff[1] = xoff ? xoff[1] : 0.0;
   hipError_t e = pf::launch_energy<T>(nx, ny, bc, E, xCr, xNb, pd, pl, chem, grad, out3,
                                       ctx->scratch, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "energy kernels");
@@ -200,6 +218,7 @@ int pf_ctx_create(pf_ctx** out) {
   if (pf_device_count() < 1) return fail(PF_ERR_NO_DEVICE, "no HIP device visible");
   pf_ctx* c = new pf_ctx();
   c->scratch = pf::Scratch{};
+  c->xoff[0] = c->xoff[1] = 0.0;
   if (hipGetDevice(&c->device) != hipSuccess) c->device = 0;
   hipError_t e = pf::scratch_alloc(c->scratch);
   if (e != hipSuccess) {
@@ -231,6 +250,15 @@ int pf_ctx_set_tile(pf_ctx* ctx, int rows, int xcd_remap, int depth) {
 }
 
 int pf_default_rows(int nx, int ny) { return pf::default_rows(nx, ny); }
+
+int pf_ctx_set_storage_offset(pf_ctx* ctx, double x_cr, double x_nb) {
+  if (!ctx) return fail(PF_ERR_ARG, "null context");
+  if (!(x_cr - x_cr == 0.0) || !(x_nb - x_nb == 0.0))
+    return fail(PF_ERR_ARG, "storage offsets must be finite");
+  ctx->xoff[0] = x_cr;
+  ctx->xoff[1] = x_nb;
+  return PF_OK;
+}
 
 int pf_step_f64(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const double* a,
                 const double* b, const double* c, const double* d, double* oa, double* ob,

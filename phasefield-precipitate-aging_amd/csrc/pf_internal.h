@@ -26,6 +26,13 @@ namespace pf {
 // q and k carry the factor m[1] instead, so that a Laplacian is
 // (S + N) + m[3] (W + E) + m[4] C: four operations instead of five.  Any
 // other mask is stored as given (scale 1) and runs the general nine-term sum.
+//
+// Stored compositions may be offset: an array holds x - xoff (FP32 storage
+// with xoff near the mean composition keeps 4-5 more bits of the small
+// per-step increments; xoff = 0 otherwise).  The step is offset-invariant but
+// for the residual x - xe_gamma of the KKS solve, so the kernels work on stored
+// values throughout with xe_g = xe_gamma - xoff; only the nucleation test
+// (which evaluates g and D at x) adds xoff back.
 struct StepParams {
   double qCr[3], qNb[3];  // scale * dt * (D row) . (A_gamma mu + xe_gamma)
   double g[2][5];         // 30 dt L * grand-potential difference  (delta, Laves)
@@ -33,6 +40,8 @@ struct StepParams {
   double c[2];            // 2 alpha dt L
   double k[2];            // scale * kappa dt L
   double m[9];            // 3x3 Laplacian mask / scale, row-major (y-1, y, y+1)
+  double xe_g[2];         // xe_gamma - xoff (Cr, Nb)
+  double xoff[2];         // composition offset of the stored arrays (Cr, Nb)
 };
 
 // The masks the folded five-point Laplacian takes (as given by the caller).
@@ -56,6 +65,7 @@ struct NucParams {
   double R_scan;                // 1.75e-9 / dx_arg
   uint64_t seed, call;
   uint32_t instance;
+  double xoff[2];  // composition offset of the stored arrays (StepParams::xoff)
 };
 
 // A nucleation event whose embedding cannot be done in place by the
@@ -74,6 +84,7 @@ struct EnergyParams {
   double dx, dy, dV;
   double kappa[2], omega[2];
   double Omega;
+  double xoff[2];  // composition offset of the stored arrays (StepParams::xoff)
 };
 
 struct Scratch {
@@ -117,14 +128,15 @@ hipError_t launch_nucleation(int nx, int ny, T* xCr, T* xNb, T* pd, T* pl, const
                              const Scratch& S, hipStream_t st);
 template <typename T>
 hipError_t launch_apply_events(int nx, int ny, T* xCr, T* xNb, T* pd, T* pl, const Scratch& S,
-                               hipStream_t st);
+                               const double* xoff, hipStream_t st);
 template <typename T>
 hipError_t launch_energy(int nx, int ny, int bc, const EnergyParams& E, const T* xCr,
                          const T* xNb, const T* pd, const T* pl, double* chem, double* grad,
                          double* out3, const Scratch& S, hipStream_t st);
 template <typename T>
 hipError_t launch_summarize(int nx, int ny, const T* xCr, const T* xNb, const T* pd,
-                            const T* pl, double* out5, const Scratch& S, hipStream_t st);
+                            const T* pl, const double* xoff, double* out5, const Scratch& S,
+                            hipStream_t st);
 hipError_t launch_dataviz(int nx, int ny, const double* xCr, const double* xNb, const double* pd,
                           const double* pl, double* conc_Ni, double* phi, hipStream_t st);
 hipError_t launch_thermo_eval(int what, int n, const double* in, double* out, hipStream_t st);
@@ -136,8 +148,10 @@ int default_rows_k(int k, int nx, int ny, int storage_bytes);
 NucParams make_nuc(double sigma_del, double sigma_lav, double unit_a, double dx, double dy,
                    double dz, double dt, uint64_t seed, uint64_t call, uint32_t instance,
                    int nx, int ny);
+// xoff: composition offset of the stored arrays (null = none)
 StepParams make_step(double dt, const double* DCr, const double* DNb, const double* kappa,
-                     const double* omega, const double* Lmob, double alpha, const double* mask);
+                     const double* omega, const double* Lmob, double alpha, const double* mask,
+                     const double* xoff = nullptr);
 
 void set_error(const std::string& msg);
 const char* last_error();

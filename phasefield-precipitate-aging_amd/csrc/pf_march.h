@@ -70,7 +70,7 @@ struct Point {       // what the neighbours of a cell need
   double qCr, qNb;   // dt D x_gamma: the Cahn-Hilliard update is x' = x + lap(q)
 };
 struct Centre {      // extra state of a cell when it is the stencil centre
-  double xCr, xNb;   // compositions
+  double xCr, xNb;   // compositions as stored (x - StepParams::xoff)
   double mu0, mu1;   // common diffusion potential
 };
 template <typename T>
@@ -132,7 +132,8 @@ __device__ __forceinline__ double p_interp6(double x) {
 // p_l (A_l - A_g) and r = x - xe_g - p_d (xe_d - xe_g) - p_l (xe_l - xe_g);
 // mu = M^-1 r, and the neighbours need q = dt D x_g, x_g = xe_g + A_g mu
 // (linear in mu; coefficients in StepParams).  The constants here carry the
-// factor 6 of p_interp6.
+// factor 6 of p_interp6.  x - xe_g is formed from the stored composition and
+// P.xe_g (xe_g less the storage offset).
 constexpr double DA_DEL[3] = {6.0 * (A[1][0] - A[0][0]), 6.0 * (A[1][1] - A[0][1]),
                               6.0 * (A[1][2] - A[0][2])};
 constexpr double DA_LAV[3] = {6.0 * (A[2][0] - A[0][0]), 6.0 * (A[2][1] - A[0][1]),
@@ -152,8 +153,8 @@ __device__ __forceinline__ void derive(const StepParams& P, const Raw<T>& r, Poi
   const double M0 = fma(pD, DA_DEL[0], fma(pL, DA_LAV[0], A[0][0]));
   const double M1 = fma(pD, DA_DEL[1], fma(pL, DA_LAV[1], A[0][1]));
   const double M2 = fma(pD, DA_DEL[2], fma(pL, DA_LAV[2], A[0][2]));
-  const double r0 = fma(-pD, DXE_DEL[0], fma(-pL, DXE_LAV[0], c.xCr - XE[0][0]));
-  const double r1 = fma(-pD, DXE_DEL[1], fma(-pL, DXE_LAV[1], c.xNb - XE[0][1]));
+  const double r0 = fma(-pD, DXE_DEL[0], fma(-pL, DXE_LAV[0], c.xCr - P.xe_g[0]));
+  const double r1 = fma(-pD, DXE_DEL[1], fma(-pL, DXE_LAV[1], c.xNb - P.xe_g[1]));
   const double inv = recip(fma(M0, M2, -(M1 * M1)));
   c.mu0 = fma(M2, r0, -(M1 * r1)) * inv;
   c.mu1 = fma(M0, r1, -(M1 * r0)) * inv;
@@ -400,12 +401,18 @@ __device__ __forceinline__ void march(
 
       const int py = 1 + y;
       if (NUC && out_lane) {
-        // decide on the stored (storage-rounded) values, as a separate pass would
-        nCr = (double)(T)nCr;
-        nNb = (double)(T)nNb;
+        // decide on the stored (storage-rounded) values, as a separate pass
+        // would, with the compositions in absolute units; an embed stores
+        // xe - xoff, anything else the stored value unchanged
+        const double sCr = (double)(T)nCr, sNb = (double)(T)nNb;
+        const double aCr = sCr + P.xoff[0], aNb = sNb + P.xoff[1];
+        double eCr = aCr, eNb = aNb;
         npd = (double)(T)npd;
         npl = (double)(T)npl;
-        nucleate_cell(N, (uint32_t)(py * pitch + 1 + gx), nCr, nNb, npd, npl, S);
+        nucleate_cell(N, (uint32_t)(py * pitch + 1 + gx), eCr, eNb, npd, npl, S);
+        const bool embedded = eCr != aCr || eNb != aNb;
+        nCr = embedded ? eCr - P.xoff[0] : sCr;
+        nNb = embedded ? eNb - P.xoff[1] : sNb;
       }
 
       if (out_lane) {

@@ -52,6 +52,7 @@ PROTOTYPES = {
     "pf_last_error": (ctypes.c_char_p, []),
     "pf_device_count": (_i, []),
     "pf_ctx_set_tile": (_i, [_vp, _i, _i, _i]),
+    "pf_ctx_set_storage_offset": (_i, [_vp, ctypes.c_double, ctypes.c_double]),
     "pf_step_f64": (_i, [_vp, _i, _i, _i, ctypes.POINTER(PfParams)] + [_vp] * 8 +
                     [ctypes.POINTER(PfNucParams), _vp]),
     "pf_step_f32": (_i, [_vp, _i, _i, _i, ctypes.POINTER(PfParams)] + [_vp] * 8 +

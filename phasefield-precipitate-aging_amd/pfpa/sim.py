@@ -48,26 +48,55 @@ class Simulation:
     """Double-buffered fields resident on the GPU.
 
     dtype torch.float64 is the plain form; torch.float32 the precision form
-    (FP32 storage, FP64 arithmetic)."""
+    (FP32 storage, FP64 arithmetic).
+
+    storage_offset (FP32 storage only): None keeps x in the arrays; "median",
+    "mean" or (x_Cr, x_Nb) stores x - offset ("median" / "mean" of the initial
+    interior compositions; the median is the matrix composition wherever the
+    matrix is most of the domain, which is where the offset pays: an FP32 ulp
+    of x - offset there is 8-100x below that of x, so each step's small
+    composition increment keeps that many more bits).  Simulations sharing a
+    context share its offset (pf_ctx_set_storage_offset).
+    fields(), summarize() and energy() report absolute compositions."""
 
     def __init__(self, setup, fields, dtype=torch.float64, nucleation=False, seed=42,
                  nuc_every=None, instance=0, stream=None, ctx=None, nuc_dx=None,
-                 steps_per_launch=None):
+                 steps_per_launch=None, storage_offset=None):
         if not torch.cuda.is_available():
             raise RuntimeError("Simulation needs a GPU")
         self.s = setup
         self.dtype = dtype
         self.nx, self.ny = setup.nx, setup.ny
         self.stream = stream if stream is not None else torch.cuda.current_stream()
+        self.xoff = (0.0, 0.0)
+        if storage_offset is not None:
+            if dtype != torch.float32:
+                raise ValueError("storage_offset applies to FP32 storage only")
+            if isinstance(storage_offset, str):
+                stat = {"median": np.median, "mean": np.mean}.get(storage_offset)
+                if stat is None:
+                    raise ValueError("storage_offset: None, 'median', 'mean' or (x_Cr, x_Nb)")
+                self.xoff = tuple(float(stat(np.asarray(f, dtype=np.float64)[1:-1, 1:-1]))
+                                  for f in fields[:2])
+            else:
+                self.xoff = (float(storage_offset[0]), float(storage_offset[1]))
         self.owns_ctx = ctx is None
         if ctx is None:
             h = ctypes.c_void_p()
             check(lib().pf_ctx_create(ctypes.byref(h)), "pf_ctx_create")
             ctx = h
         self.ctx = ctx
+        if dtype == torch.float32:
+            check(lib().pf_ctx_set_storage_offset(ctx, self.xoff[0], self.xoff[1]),
+                  "pf_ctx_set_storage_offset")
+
+        def stored(k, f):
+            off = self.xoff[k] if k < 2 else 0.0
+            return np.ascontiguousarray(np.asarray(f, dtype=np.float64) - off if off else f)
+
         with torch.cuda.stream(self.stream):
-            self.a = [torch.as_tensor(np.ascontiguousarray(f)).to(device="cuda", dtype=dtype)
-                      for f in fields]
+            self.a = [torch.as_tensor(stored(k, f)).to(device="cuda", dtype=dtype)
+                      for k, f in enumerate(fields)]
             self.b = [torch.empty_like(x) for x in self.a]
         P = PfParams()
         P.dt = setup.dt
@@ -180,8 +209,13 @@ class Simulation:
             self.steps_done += done
 
     def fields(self):
+        """The current fields as FP64 arrays, compositions in absolute units."""
         self.stream.synchronize()
-        return [x.reshape(self.ny, self.nx).double().cpu().numpy() for x in self.a]
+        out = [x.reshape(self.ny, self.nx).double().cpu().numpy() for x in self.a]
+        for k in range(2):
+            if self.xoff[k] != 0.0:
+                out[k] = out[k] + self.xoff[k]
+        return out
 
     def summarize(self):
         out = torch.zeros(5, dtype=torch.float64, device="cuda")
