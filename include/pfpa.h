@@ -231,6 +231,8 @@ typedef struct pf_job_desc {
 	uint32_t instance;     /* folded into the random counter: unique per simulation */
 	int nuc_every;
 	double sigma_del, sigma_lav, unit_a;
+	double x_offset[2];    /* FP32 storage: the job's Cr / Nb arrays hold x - x_offset
+	                          (pf_ctx_set_storage_offset); 0 = absolute.  FP64: must be 0 */
 } pf_job_desc;
 
 typedef struct pf_batch pf_batch;
@@ -248,7 +250,8 @@ int pf_batch_set_tile(pf_batch* batch, int rows, int depth);
  * remain or where a nucleation step falls before the last of them (the
  * nucleation pass runs after the launch's last step). */
 int pf_batch_set_launch_steps(pf_batch* batch, int steps);
-/* Admit a job; fields are HOST arrays of the batch's storage type, (Nx+2)*(Ny+2) each.
+/* Admit a job; fields are HOST arrays of the batch's storage type, (Nx+2)*(Ny+2) each
+ * (compositions less desc->x_offset).
  * Returns PF_ERR_OVERFLOW when the pool has no room (the caller may retry later). */
 int pf_batch_add(pf_batch* batch, const pf_job_desc* desc, const void* conc_Cr,
                  const void* conc_Nb, const void* phi_del, const void* phi_lav, int* job_id);
@@ -266,10 +269,12 @@ int pf_batch_sync(pf_batch* batch);
 int pf_batch_poll(pf_batch* batch, int job_id, uint64_t* steps_done, int* active);
 int pf_batch_poll_all(pf_batch* batch, int capacity, int* job_ids, uint64_t* steps_done,
                       int* active, int* count);
-/* Synchronous copy of a job's current fields to host arrays (NULL skips a field). */
+/* Synchronous copy of a job's current fields to host arrays (NULL skips a field),
+ * as stored (compositions less the job's x_offset). */
 int pf_batch_read(pf_batch* batch, int job_id, void* conc_Cr, void* conc_Nb, void* phi_del,
                   void* phi_lav);
-/* Synchronous per-job means {x_Cr, x_Nb, gamma, delta, Laves} (compensated FP64). */
+/* Synchronous per-job means {x_Cr, x_Nb, gamma, delta, Laves} (compensated FP64;
+ * compositions absolute). */
 int pf_batch_summarize(pf_batch* batch, int job_id, double* out5);
 /* Fails if any nucleation event needed an embed wider than one cell. */
 int pf_batch_status(pf_batch* batch, uint64_t* wide_events);

@@ -158,7 +158,8 @@ __global__ __launch_bounds__(WAVE * WPB) void batch_nuc_kernel(
     const int y0 = chunk * J.rows, y1 = min(y0 + J.rows, J.Ny);
     for (int y = y0; y < y1; y++) {
       const size_t o = (size_t)(1 + y) * J.pitch + 1 + gx;
-      double xCr = (double)out[o], xNb = (double)out[o + pl];
+      // absolute compositions (the job's arrays hold x - xoff)
+      double xCr = (double)out[o] + J.P.xoff[0], xNb = (double)out[o + pl] + J.P.xoff[1];
       double pd = (double)out[o + 2 * pl], plv = (double)out[o + 3 * pl];
       const double c0 = xCr, c1 = xNb, c2 = pd, c3 = plv;
       Scratch S{};
@@ -166,6 +167,8 @@ __global__ __launch_bounds__(WAVE * WPB) void batch_nuc_kernel(
       S.event_capacity = 0;
       nucleate_cell(N, (uint32_t)((1 + y) * J.pitch + 1 + gx), xCr, xNb, pd, plv, S);
       if (xCr != c0 || xNb != c1 || pd != c2 || plv != c3) {
+        xCr -= J.P.xoff[0];
+        xNb -= J.P.xoff[1];
         store4(out, out + pl, out + 2 * pl, out + 3 * pl, o, xCr, xNb, pd, plv);
         if (J.bc == 0)
           store_ghost_images<T, 0>(out, out + pl, out + 2 * pl, out + 3 * pl, gx, y, J.Nx, J.Ny,
@@ -261,7 +264,7 @@ __global__ void batch_final_kernel(const BatchJob* __restrict__ jobs, const doub
   const BatchJob& J = jobs[job];
   KahanB a{0.0, 0.0};
   for (int r = 0; r < BATCH_RB; r++) kb_merge(a, KahanB{partials[r * 10 + 2 * k], partials[r * 10 + 2 * k + 1]});
-  out5[k] = (a.s + a.c) / ((double)J.Nx * (double)J.Ny);
+  out5[k] = (a.s + a.c) / ((double)J.Nx * (double)J.Ny) + (k < 2 ? J.P.xoff[k] : 0.0);
 }
 
 }  // namespace pf
@@ -499,6 +502,12 @@ int pf_batch_add(pf_batch* b, const pf_job_desc* d, const void* xCr, const void*
   if (d->Nx < 1 || d->Ny < 1 || (long long)(d->Nx + 2) * (d->Ny + 2) >= (1LL << 31))
     return bfail(PF_ERR_ARG, "bad grid size");
   if (d->bc != PF_BC_NOFLUX && d->bc != PF_BC_PERIODIC) return bfail(PF_ERR_ARG, "bad boundary mode");
+  for (int c = 0; c < 2; c++) {
+    if (!(d->x_offset[c] - d->x_offset[c] == 0.0))
+      return bfail(PF_ERR_ARG, "x_offset must be finite");
+    if (b->storage == PF_STORAGE_F64 && d->x_offset[c] != 0.0)
+      return bfail(PF_ERR_ARG, "x_offset applies to FP32 storage only");
+  }
   const double* m = d->prm.mask;
   if (!pf::symmetric_five_point(m))
     return bfail(PF_ERR_ARG,
@@ -541,7 +550,7 @@ int pf_batch_add(pf_batch* b, const pf_job_desc* d, const void* xCr, const void*
   J.active = d->steps > 0 ? 1 : 0;
   J.nuc_every = d->nucleation ? d->nuc_every : 0;
   J.P = pf::make_step(d->prm.dt, d->prm.DCr, d->prm.DNb, d->prm.kappa, d->prm.omega,
-                      d->prm.Lmob, d->prm.alpha, d->prm.mask);
+                      d->prm.Lmob, d->prm.alpha, d->prm.mask, d->x_offset);
   J.N = pf::make_nuc(d->sigma_del, d->sigma_lav, d->unit_a, d->dx, d->dx, d->dx,
                      (double)(d->nuc_every > 0 ? d->nuc_every : 1) * d->prm.dt, d->seed, 0,
                      d->instance, d->Nx + 2, d->Ny + 2);

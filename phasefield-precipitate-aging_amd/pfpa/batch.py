@@ -18,7 +18,7 @@ class PfJobDesc(ctypes.Structure):
                 ("steps", ctypes.c_uint64), ("seed", ctypes.c_uint64),
                 ("instance", ctypes.c_uint32), ("nuc_every", ctypes.c_int),
                 ("sigma_del", ctypes.c_double), ("sigma_lav", ctypes.c_double),
-                ("unit_a", ctypes.c_double)]
+                ("unit_a", ctypes.c_double), ("x_offset", ctypes.c_double * 2)]
 
 
 _vp = ctypes.c_void_p
@@ -61,7 +61,11 @@ def job_bytes(storage, Nx, Ny):
     return blib().pf_batch_job_bytes(storage, Nx, Ny)
 
 
-def make_desc(setup, steps, nucleation=False, seed=42, instance=0, nuc_every=None, nuc_dx=None):
+def make_desc(setup, steps, nucleation=False, seed=42, instance=0, nuc_every=None, nuc_dx=None,
+              x_offset=(0.0, 0.0)):
+    """A job descriptor.  x_offset (FP32 storage): the job's composition arrays
+    hold x - x_offset (Batch.add / read convert; see Simulation's
+    storage_offset)."""
     d = PfJobDesc()
     d.Nx, d.Ny = setup.N
     d.bc = setup.bc
@@ -84,6 +88,7 @@ def make_desc(setup, steps, nucleation=False, seed=42, instance=0, nuc_every=Non
     d.nuc_every = int(nuc_every if nuc_every else max(1, int(0.0001 / setup.dt)))
     d.sigma_del, d.sigma_lav = SIGMA
     d.unit_a = LATTICE_CONST
+    d.x_offset[0], d.x_offset[1] = float(x_offset[0]), float(x_offset[1])
     return d
 
 
@@ -96,6 +101,7 @@ class Batch:
               "pf_batch_create")
         self.h = h
         self.shapes = {}
+        self.offsets = {}
         self.launch_steps = 1
 
     def set_tile(self, rows, depth=3):
@@ -107,8 +113,11 @@ class Batch:
         self.launch_steps = steps
 
     def add(self, desc, fields):
-        """Admit a job; returns its id, or None if the pool is full."""
-        arrs = [np.ascontiguousarray(f, dtype=self.dtype) for f in fields]
+        """Admit a job (fields in absolute compositions); returns its id, or
+        None if the pool is full."""
+        off = (desc.x_offset[0], desc.x_offset[1], 0.0, 0.0)
+        arrs = [np.ascontiguousarray(np.asarray(f, dtype=np.float64) - off[k] if off[k] else f,
+                                     dtype=self.dtype) for k, f in enumerate(fields)]
         ny, nx = arrs[0].shape
         if (nx, ny) != (desc.Nx + 2, desc.Ny + 2):
             raise ValueError("field shape %s does not match the job size" % (arrs[0].shape,))
@@ -119,11 +128,13 @@ class Batch:
             return None
         check(rc, "pf_batch_add")
         self.shapes[jid.value] = (ny, nx)
+        self.offsets[jid.value] = off[:2]
         return jid.value
 
     def remove(self, jid):
         check(blib().pf_batch_remove(self.h, jid), "pf_batch_remove")
         self.shapes.pop(jid, None)
+        self.offsets.pop(jid, None)
 
     def commit(self):
         check(blib().pf_batch_commit(self.h), "pf_batch_commit")
@@ -153,11 +164,18 @@ class Batch:
               "poll_all")
         return [(ids[k], st[k], bool(ac[k])) for k in range(min(n.value, capacity))]
 
-    def read(self, jid):
+    def read(self, jid, stored=False):
+        """A job's current fields; compositions absolute (as FP64 when the job
+        has an offset) unless `stored`."""
         ny, nx = self.shapes[jid]
         out = [np.empty((ny, nx), dtype=self.dtype) for _ in range(4)]
         check(blib().pf_batch_read(self.h, jid, *[a.ctypes.data_as(_vp) for a in out]),
               "pf_batch_read")
+        off = self.offsets.get(jid, (0.0, 0.0))
+        if not stored:
+            for k in range(2):
+                if off[k]:
+                    out[k] = out[k].astype(np.float64) + off[k]
         return out
 
     def summarize(self, jid):

@@ -157,6 +157,41 @@ def test_batch_nucleation_matches_plain(gpu, launch_steps):
             assert np.array_equal(got[k].astype(np.float64), want[k]), (storage, k)
 
 
+@pytest.mark.parametrize("launch_steps", [1, 3])
+def test_batch_offset_matches_plain(gpu, launch_steps):
+    """FP32 jobs storing x - x_offset (with the nucleation schedule and P in
+    (0, 1)) hold the same bits as the plain FP32 path with the same storage
+    offset; reads and summaries come back in absolute compositions."""
+    off = (0.22, 0.21)
+    s = M.make_setup(64, 48, bc=1)
+    f = nuc_fields(64, 48, 1, 3)
+    b = Batch(PF_STORAGE_F32, pool_bytes=64 << 20)
+    b.set_launch_steps(launch_steps)
+    kw = dict(nucleation=True, seed=123, instance=4, nuc_every=3, nuc_dx=NUC_DX)
+    j = b.add(make_desc(s, 7, x_offset=off, **kw), f)
+    b.run(7)
+    assert b.poll(j) == (7, False)
+    raw, got, summ = b.read(j, stored=True), b.read(j), b.summarize(j)
+    b.close()
+    sim = Simulation(s, f, dtype=torch.float32, storage_offset=off, **kw)
+    sim.step(7)
+    want_raw = [x.reshape(s.ny, s.nx).cpu().numpy() for x in sim.a]
+    want, want_summ = sim.fields(), sim.summarize()
+    sim.close()
+    nucleated = (want[2] == 1.0) | (want[3] == 1.0)
+    assert nucleated[1:-1, 1:-1].sum() > 50
+    for k in range(4):
+        assert np.array_equal(raw[k], want_raw[k]), k
+        assert np.array_equal(np.asarray(got[k], dtype=np.float64), want[k]), k
+    for k in range(5):
+        assert summ[k] == pytest.approx(want_summ[k], rel=1e-12, abs=1e-15), k
+    # FP64 storage takes no offset
+    b = Batch(PF_STORAGE_F64, pool_bytes=64 << 20)
+    with pytest.raises(Exception):
+        b.add(make_desc(s, 7, x_offset=off), f)
+    b.close()
+
+
 def test_admission_by_pool_budget(gpu):
     from pfpa.batch import job_bytes
     per = job_bytes(PF_STORAGE_F32, 128, 128)
