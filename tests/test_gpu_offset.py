@@ -191,6 +191,64 @@ def test_offset_accuracy_and_reports(gpu):
         sim.close()
 
 
+def test_offset_wide_nucleation_events(gpu, oracle):
+    """The deferred-event path (scan over the whole grid, overlapping embeds of
+    radius ~1.7 cells, gamma rings around them) on offset storage: the oracle's
+    result on the absolute compositions, less the offset."""
+    Nx, Ny = 38, 30
+    nx, ny = Nx + 2, Ny + 2
+    rng = np.random.default_rng(21)
+    absolute = [M.fill_ghosts(rng.uniform(0.15, 0.3, (ny, nx)), 0),
+                M.fill_ghosts(rng.uniform(0.19, 0.24, (ny, nx)), 0),
+                np.zeros((ny, nx)), np.zeros((ny, nx))]
+    stored = [(absolute[k] - OFF[k]).astype(np.float32) for k in range(2)]
+    stored += [f.astype(np.float32) for f in absolute[2:]]
+    absolute = [stored[k].astype(np.float64) + OFF[k] for k in range(2)] + absolute[2:]
+    a = dict(sigma_del=0.0105, sigma_lav=0.0108, unit_a=0.352e-9, dx=2e-13, dz=2e-13, dt=3e38)
+    want, nev, _ = oracle.nucleation(absolute, a["sigma_del"], a["sigma_lav"], a["unit_a"],
+                                     a["dx"] * nx, a["dx"] * ny, a["dz"], a["dt"],
+                                     seed=77, call=2)
+    assert nev > 20
+    ctx = ctypes.c_void_p()
+    check(lib().pf_ctx_create(ctypes.byref(ctx)), "ctx")
+    check(lib().pf_ctx_set_storage_offset(ctx, *OFF), "offset")
+    d = [_dev(x, torch.float32) for x in stored]
+    N = _nuc_struct(a, 77, 2)
+    check(lib().pf_nucleation_f32(ctx, nx, ny, *[_p(x) for x in d], ctypes.byref(N), None),
+          "nucleation")
+    torch.cuda.synchronize()
+    for k in range(4):
+        expect = (want[k] - (OFF[k] if k < 2 else 0.0)).astype(np.float32)
+        assert np.array_equal(d[k].cpu().numpy(), expect), k
+    wide = ctypes.c_uint64()
+    over = ctypes.c_int()
+    check(lib().pf_nucleation_status(ctx, ctypes.byref(wide), ctypes.byref(over)), "status")
+    assert wide.value == nev and over.value == 0
+    lib().pf_ctx_destroy(ctx)
+
+
+def test_offset_nine_point_mask(gpu):
+    """The general nine-term Laplacian (one step per launch) on offset
+    storage: within FP32 rounding of the same run without an offset."""
+    Nx, Ny = 96, 70
+    s = M.make_setup(Nx, Ny, bc=0, dt_rule=5e-8)
+    s.mask = M.nine_point_mask(s.dx, 1.3 * s.dx)
+    f0 = M.init_matrix_with_particles(Nx, Ny, s.dx, 0, seed=8, radius=2e-9)
+    a = Simulation(s, f0, dtype=torch.float32, storage_offset="median")
+    b = Simulation(s, f0, dtype=torch.float32)
+    c = Simulation(s, f0, dtype=torch.float64)
+    assert a.steps_per_launch == 1
+    for sim in (a, b, c):
+        sim.step(25)
+    fa, fb, fc = a.fields(), b.fields(), c.fields()
+    for k in range(4):
+        ea = np.abs(fa[k] - fc[k])[1:-1, 1:-1].max()
+        eb = np.abs(fb[k] - fc[k])[1:-1, 1:-1].max()
+        assert ea <= max(eb, 1e-7), (k, ea, eb)
+    for sim in (a, b, c):
+        sim.close()
+
+
 def test_offset_errors(gpu):
     ctx = ctypes.c_void_p()
     check(lib().pf_ctx_create(ctypes.byref(ctx)), "ctx")
