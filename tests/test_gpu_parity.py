@@ -381,6 +381,61 @@ def test_mass_conservation_4096(gpu):
     sim.close()
 
 
+def test_largest_grid_fp32(gpu):
+    """The largest square grid the 32-bit cell index allows (46340^2 with the
+    ghost ring, 2.1e9 cells; FP32 storage, 69 GB of fields): a perturbation
+    at the last interior cell, whose periodic neighbours wrap to the start of
+    the arrays, evolves over a three-step and a one-step launch exactly as the
+    same perturbation at the corner of a 64^2 grid."""
+    def run(n, pos):
+        s = M.make_setup(n - 2, n - 2, bc=1)
+        P = PfParams()
+        P.dt = s.dt
+        for k in range(2):
+            P.DCr[k], P.DNb[k] = s.DCr[k], s.DNb[k]
+            P.kappa[k], P.omega[k], P.Lmob[k] = s.kappa[k], s.omega[k], s.Lmob[k]
+        P.alpha = s.alpha
+        for k in range(9):
+            P.mask[k] = s.mask[k]
+        vals = (0.30, 0.02, 0.0, 0.0)
+        a = [torch.full((n * n,), v, dtype=torch.float32, device="cuda") for v in vals]
+        y, x = pos
+        bump = (0.05, 0.01, 0.3, 0.2)
+        for k in range(4):
+            a[k].view(n, n)[y, x] += bump[k]
+            a[k].view(n, n)[0, :] = a[k].view(n, n)[n - 2, :]   # periodic ghost ring
+            a[k].view(n, n)[n - 1, :] = a[k].view(n, n)[1, :]
+            a[k].view(n, n)[:, 0] = a[k].view(n, n)[:, n - 2]
+            a[k].view(n, n)[:, n - 1] = a[k].view(n, n)[:, 1]
+        b = [torch.empty_like(t) for t in a]
+        ctx = ctypes.c_void_p()
+        check(lib().pf_ctx_create(ctypes.byref(ctx)), "ctx")
+        check(lib().pf_step3_f32(ctx, n, n, 1, ctypes.byref(P), *map(_p, a), *map(_p, b),
+                                 None), "step3")
+        check(lib().pf_step_f32(ctx, n, n, 1, ctypes.byref(P), *map(_p, b), *map(_p, a),
+                                None, None), "step")
+        torch.cuda.synchronize()
+        lib().pf_ctx_destroy(ctx)
+        # the 11 x 11 interior neighbourhood of the last interior cell, wrapped
+        idx = [(n - 2 - 5 + i - 1) % (n - 2) + 1 for i in range(11)]
+        rows = torch.tensor(idx, device="cuda")
+        out = [t.view(n, n).index_select(0, rows).index_select(1, rows).cpu().numpy()
+               for t in a]
+        far = [float(t.view(n, n)[(n - 2) // 2, (n - 2) // 2]) for t in a]
+        del a, b
+        torch.cuda.empty_cache()
+        return out, far
+
+    big = 46340
+    assert big * big < 2 ** 31 <= (big + 1) * (big + 1)
+    small_out, small_far = run(64, (62, 62))
+    big_out, big_far = run(big, (big - 2, big - 2))
+    for k in range(4):
+        assert np.array_equal(big_out[k], small_out[k]), k
+        assert big_far[k] == small_far[k], k
+    assert np.abs(small_out[2] - small_out[2][0, 0]).max() > 0   # the bump has spread
+
+
 def test_precision_form_8192(gpu):
     """The precision workload's size (8192^2, the plain workload's
     initialisation, periodic): FP32 storage and FP64 from the same fields for
