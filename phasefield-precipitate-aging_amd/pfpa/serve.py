@@ -71,12 +71,23 @@ def request_desc(r, dx=0.25e-9):
     return s, make_desc(s, r.steps, nucleation=True, seed=r.seed, instance=r.rid)
 
 
+def request_offset(r):
+    """The composition offset a request's FP32 arrays are stored relative to:
+    its matrix composition (known from the request, so admission computes
+    nothing for it)."""
+    return (float(r.x0[0]), float(r.x0[1]))
+
+
 def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_steps=3,
-          log=None, storage=PF_STORAGE_F32, time_limit=None, read_fields=False):
+          log=None, storage=PF_STORAGE_F32, time_limit=None, read_fields=False,
+          offset=True):
     """Run the request stream; returns (finished requests, stats).  Each device
     round is one replay of a graph of `launches_per_graph` step launches, each
     advancing every running job by up to `launch_steps` steps (1, 2 or 3).
-    read_fields: also keep each request's final fields (r.fields)."""
+    read_fields: also keep each request's final fields (r.fields, absolute
+    compositions).  offset (FP32 storage): store each request's compositions
+    relative to its matrix composition (request_offset), which keeps more bits
+    of each step's increment."""
     batch = Batch(storage, pool_bytes=pool_bytes, max_jobs=max(16, len(requests)))
     batch.set_tile(rows, depth)
     batch.set_launch_steps(launch_steps)
@@ -91,7 +102,10 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
     def prepare(r):
         if r.rid not in prepared:
             s, d = request_desc(r)
-            prepared[r.rid] = (d, request_fields(r))
+            f = request_fields(r)
+            if offset and storage == PF_STORAGE_F32:
+                d.x_offset[0], d.x_offset[1] = request_offset(r)
+            prepared[r.rid] = (d, f)
         return prepared[r.rid]
 
     while pending or waiting or running:

@@ -6,7 +6,7 @@ import pytest
 import torch
 
 from pfpa.batch import PF_STORAGE_F32, job_bytes
-from pfpa.serve import make_requests, request_desc, request_fields, serve
+from pfpa.serve import make_requests, request_desc, request_fields, request_offset, serve
 from pfpa.sim import Simulation
 
 pytestmark = pytest.mark.gpu
@@ -31,10 +31,12 @@ def test_serving_alone_vs_mix():
         for k in range(4):
             assert np.array_equal(a.fields[k], m.fields[k]), (rid, k)
         assert a.summary == m.summary
-        # and the plain FP32-storage path alone
+        # and the plain FP32-storage path alone, with the same composition offset
+        # (the serving loop stores each request relative to its matrix composition)
         s, _ = request_desc(r0)
+        assert request_offset(r0) == (r0.x0[0], r0.x0[1]) and r0.x0[0] > 0.2
         sim = Simulation(s, request_fields(r0), dtype=torch.float32, nucleation=True,
-                         seed=r0.seed, instance=r0.rid)
+                         seed=r0.seed, instance=r0.rid, storage_offset=request_offset(r0))
         sim.step(r0.steps)
         want = sim.fields()
         sim.close()
