@@ -35,3 +35,26 @@ def test_bench_json_line():
     assert rf["frac"] == pytest.approx(rf["achieved"] / rf["peak"], rel=1e-12)
     assert d["cpu_baseline"] is None
 
+
+
+def test_bench_precision_form_line():
+    """bench.py --form precision: one JSON line with the error rows at each mark
+    for FP32 storage with and without the composition offset, and the rates."""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--form", "precision",
+                        "--size", "256", "--precision-marks", "10", "30"],
+                       capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = [l for l in r.stdout.splitlines() if l.strip()]
+    assert len(lines) == 1
+    d = json.loads(lines[0])
+    assert d["form"] == "precision" and d["grid"] == [256, 256]
+    assert [row["step"] for row in d["rows"]] == [10, 30]
+    for row in d["rows"]:
+        for key in ("max_abs_dx_Cr", "max_abs_dx_Nb", "max_abs_dx_Cr_offset",
+                    "max_abs_dx_Nb_offset", "mass_drift_Cr_32", "mass_drift_Cr_32_offset"):
+            assert key in row, key
+        assert 0 < row["max_abs_dx_Cr_offset"] < row["max_abs_dx_Cr"] < 1e-5
+    rates = d["rates"]
+    for key in ("fp64", "fp32_storage", "fp32_storage_offset"):
+        assert rates[key] > 0, key
+    assert rates["steps"] == 30 and len(rates["offset"]) == 2
