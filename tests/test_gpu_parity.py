@@ -14,7 +14,7 @@ import torch
 import pfpa
 from pfpa import model as M
 from pfpa._lib import PfNucParams, PfParams, check, lib
-from pfpa.sim import Simulation
+from pfpa.sim import Simulation, iter_launches
 
 pytestmark = pytest.mark.gpu
 
@@ -379,6 +379,24 @@ def test_mass_conservation_4096(gpu):
     assert m1[0] == pytest.approx(m0[0], rel=1e-13)
     assert m1[1] == pytest.approx(m0[1], rel=1e-13)
     sim.close()
+
+
+def test_parity_4096_benchmark_workload(gpu, oracle):
+    """The benchmark's own workload (4096^2 periodic, enriched band, nucleation
+    on: step 0 is a nucleation step, whose P rounds to 0) for 8 steps, which the
+    stepping loop runs as one fused nucleation launch, two three-step launches
+    and one one-step launch: every cell against the oracle."""
+    s = M.make_setup(4096, 4096, bc=1)
+    f0 = M.init_enriched_band(4096, 4096, 1, seed=42)
+    sim = Simulation(s, f0, nucleation=True)
+    assert list(iter_launches(0, 8, 3, sim.nuc_every)) == [(1, True), (3, False), (3, False),
+                                                          (1, False)]
+    sim.step(8)
+    got = sim.fields()
+    sim.close()
+    want = oracle.run(f0, oracle_params(oracle, s), 1, 8)
+    for k in range(4):
+        assert mixed_err(got[k], want[k]) <= TOL, k
 
 
 def test_largest_grid_fp32(gpu):
