@@ -97,6 +97,21 @@ def test_cpp_driver_replay_boundary_and_mask(gpu, oracle, bc, mask_code):
         assert mixed_err(a[k], want[k]) <= TOL, k
 
 
+def test_cpp_driver_replay_4096(gpu, oracle):
+    """The drop-in benchmark's grid: the reference's call sequence on 4096^2
+    (no-flux, the enriched-band workload, nucleation every 891 steps, I/O at
+    the last step), 7 steps grouped three to a launch, every cell against the
+    oracle."""
+    assert os.path.exists(HARNESS), "tests/cpp/replay_main not built: run __graft_entry__.build()"
+    s = M.make_setup(4096, 4096, bc=0)
+    f0 = M.init_enriched_band(4096, 4096, 0, seed=42)
+    steps = 7
+    arr = run_harness(f0, s, steps, 891, steps)
+    want = oracle_reference(oracle, f0, s, steps)
+    for k in range(4):
+        assert mixed_err(arr[k], want[k]) <= TOL, k
+
+
 @pytest.mark.parametrize("io", [1, 2, 7])
 def test_cpp_driver_replay_frequent_io(gpu, io):
     """I/O (device_dataviz + read_out_result) every 1, 2 or 7 steps breaks the
