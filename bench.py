@@ -110,6 +110,24 @@ def cpu_baseline(N, steps, warmup=1):
     return N * N * steps / el, O.num_threads(), el
 
 
+def cpu_baseline_single(N, steps):
+    """The same on one thread (a shorter sample); restores the thread count."""
+    import oracle as O
+    from pfpa import model as M
+    s = M.make_setup(N, N, bc=1)
+    f0 = M.init_enriched_band(N, N, 1, seed=42)
+    P = O.make_params(s.dt, s.DCr, s.DNb, s.kappa, s.omega, s.Lmob, s.alpha, s.mask)
+    threads = O.num_threads()
+    O.set_threads(1)
+    try:
+        t0 = time.perf_counter()
+        O.run(f0, P, 1, steps)
+        el = time.perf_counter() - t0
+    finally:
+        O.set_threads(threads)
+    return N * N * steps / el, el
+
+
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
@@ -589,6 +607,10 @@ def main():
                "sample": "%d steps of the %dx%d FP64 plain step (same init, periodic) after "
                          "1 warm-up step, %.1f s" % (args.cpu_steps, cpu_N, cpu_N, el),
                "host_cpu": host_cpu()}
+        v1, el1 = cpu_baseline_single(cpu_N, 2)
+        cpu["single_thread"] = {"value": v1, "unit": "cell-updates/s", "cores": 1,
+                                "sample": "2 steps of the same %dx%d step, %.1f s"
+                                          % (cpu_N, cpu_N, el1)}
     out["cpu_baseline"] = cpu
     out["bench_wall_s"] = time.perf_counter() - t_all
     print(json.dumps(out))

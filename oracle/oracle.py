@@ -91,6 +91,8 @@ def lib():
         L.orc_energy.argtypes = [_i, _i, _i, _d, _d, _dp, _dp, _d] + [_dp] * 7
         L.orc_num_threads.restype = _i
         L.orc_num_threads.argtypes = []
+        L.orc_set_threads.restype = None
+        L.orc_set_threads.argtypes = [_i]
         _lib = L
     return _lib
 
@@ -244,6 +246,11 @@ def summarize(fields):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def set_threads(n):
+    """Threads of the oracle's parallel loops from now on."""
+    lib().orc_set_threads(int(n))
 
 
 # ------------------------------------------------ reference thermo (compiled)

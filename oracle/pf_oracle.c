@@ -649,3 +649,13 @@ int orc_num_threads(void)
 	return 1;
 #endif
 }
+
+/* Threads of the following parallel loops (the CPU baseline's single-core leg). */
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+	if (n > 0) omp_set_num_threads(n);
+#else
+	(void)n;
+#endif
+}
