@@ -397,9 +397,16 @@ def dropin_section(args):
 def serving_section(args):
     from pfpa.serve import fp64_replay_error, make_requests, serve
     reqs = make_requests(args.serve_rate, args.serve_duration, seed=1)
+    done = [0]
+
+    def progress(r):   # one line per 100 completions (long runs stay visibly alive)
+        done[0] += 1
+        if done[0] % 100 == 0:
+            log("serving: %d of %d requests done" % (done[0], len(reqs)))
+
     finished, stats = serve(reqs, pool_bytes=args.serve_pool_gb << 30,
                             launches_per_graph=args.serve_lpg, rows=args.serve_rows,
-                            depth=3, launch_steps=3)
+                            depth=3, launch_steps=3, log=progress)
     sample = sorted(finished, key=lambda r: r.n * r.n * r.steps)[:3]
     err, _ = fp64_replay_error(finished, sample)
     stats["arrival_rate_per_s"] = args.serve_rate
