@@ -215,18 +215,27 @@ def init_matrix_with_particles(Nx, Ny, dx, bc, seed=1, xCr0=0.3041, xNb0=0.0235,
     R = radius / dx
     dels, cs = _place_discs(rng, Nx, Ny, R, n_del, 2 * R + 8)
     lavs, cs = _place_discs(rng, Nx, Ny, R, n_lav, 2 * R + 8, cs)
-    yy, xx = np.mgrid[0:Ny, 0:Nx]
+
+    def box(c, n):
+        # the interior indices within R + 1 of centre c (wrapped when periodic)
+        i = np.arange(int(math.floor(c - R)) - 1, int(math.ceil(c + R)) + 2)
+        return np.unique(i % n) if bc == 1 else i[(i >= 0) & (i < n)]
+
     for centres, phi, ph in ((dels, pd, 1), (lavs, pl, 2)):
         for (cx, cy) in centres:
-            ddx = np.abs(xx + 0.5 - cx)
-            ddy = np.abs(yy + 0.5 - cy)
+            xs, ys = box(cx, Nx), box(cy, Ny)
+            # per cell the same arithmetic as over the whole grid, on the box only
+            ddx = np.abs(xs[None, :] + 0.5 - cx)
+            ddy = np.abs(ys[:, None] + 0.5 - cy)
             if bc == 1:
                 ddx = np.minimum(ddx, Nx - ddx)
                 ddy = np.minimum(ddy, Ny - ddy)
             inside = np.hypot(ddx, ddy) <= R
-            phi[1:-1, 1:-1][inside] = 1.0
-            xCr[1:-1, 1:-1][inside] = _xe(ph, 0)
-            xNb[1:-1, 1:-1][inside] = _xe(ph, 1)
+            rows, cols = np.broadcast_arrays(ys[:, None] + 1, xs[None, :] + 1)
+            rows, cols = rows[inside], cols[inside]
+            phi[rows, cols] = 1.0
+            xCr[rows, cols] = _xe(ph, 0)
+            xNb[rows, cols] = _xe(ph, 1)
     for a in (xCr, xNb, pd, pl):
         fill_ghosts(a, bc)
     return [xCr, xNb, pd, pl]

@@ -11,6 +11,7 @@ is measured from arrival to retirement.
 """
 import math
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -95,17 +96,24 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
     waiting = []
     running = {}                   # job id -> request
     finished = []
-    prepared = {}                  # rid -> (desc, fields), built while the GPU runs
+    rejected = []                  # larger than the whole pool
+    # rid -> future of (desc, fields), built on a worker thread while the loop
+    # keeps the device fed (a 2048^2 request's fields take ~0.2 s on the host)
+    prepared = {}
+    workers = ThreadPoolExecutor(max_workers=1)
     cell_updates = 0
     t0 = time.perf_counter()
 
+    def build(r):
+        s, d = request_desc(r)
+        f = request_fields(r)
+        if offset and storage == PF_STORAGE_F32:
+            d.x_offset[0], d.x_offset[1] = request_offset(r)
+        return d, f
+
     def prepare(r):
         if r.rid not in prepared:
-            s, d = request_desc(r)
-            f = request_fields(r)
-            if offset and storage == PF_STORAGE_F32:
-                d.x_offset[0], d.x_offset[1] = request_offset(r)
-            prepared[r.rid] = (d, f)
+            prepared[r.rid] = workers.submit(build, r)
         return prepared[r.rid]
 
     while pending or waiting or running:
@@ -114,13 +122,24 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
             break
         while pending and pending[0].arrival <= now:
             waiting.append(pending.pop(0))
-        # admit in arrival order; a request that does not fit lets smaller ones pass
+        # admit in arrival order; a request that does not fit lets smaller ones
+        # pass, and one whose fields are still being built waits for a later
+        # round while the device has work
         still = []
         for r in waiting:
-            d, f = prepare(r)
+            fut = prepare(r)
+            if running and not fut.done():
+                still.append(r)
+                continue
+            d, f = fut.result()
             jid = batch.add(d, f)
             if jid is None:
-                still.append(r)
+                if not running:
+                    # the pool is empty and still too small: it never fits
+                    del prepared[r.rid]
+                    rejected.append(r)
+                else:
+                    still.append(r)
                 continue
             del prepared[r.rid]
             r.job = jid
@@ -129,9 +148,9 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
         waiting = still
         if running:
             batch.run(launches_per_graph, launches_per_graph)   # asynchronous
-            # build the next admissions' fields while the device works
+            # start building the next admissions' fields
             now = time.perf_counter() - t0
-            for r in waiting[:4] + [q for q in pending[:4] if q.arrival <= now + 0.05]:
+            for r in waiting[:4] + [q for q in pending[:8] if q.arrival <= now + 0.5]:
                 prepare(r)
             states = batch.poll_all()
             for jid, steps, active in states:
@@ -153,12 +172,14 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
             prepare(pending[0])
             time.sleep(max(0.0, min(0.01, pending[0].arrival - (time.perf_counter() - t0))))
     wall = time.perf_counter() - t0
+    workers.shutdown(wait=True, cancel_futures=True)
     batch.status()
     batch.close()
     lat = np.array([r.done - r.arrival for r in finished]) if finished else np.zeros(1)
     stats = {
         "requests": len(requests),
         "completed": len(finished),
+        "rejected": len(rejected),
         "wall_s": wall,
         "cell_updates": cell_updates,
         "throughput_cell_updates_per_s": cell_updates / wall if wall > 0 else 0.0,

@@ -42,3 +42,17 @@ def test_serving_alone_vs_mix():
         sim.close()
         for k in range(4):
             assert np.array_equal(m.fields[k].astype(np.float64), want[k]), (rid, k)
+
+
+def test_request_larger_than_the_pool_is_rejected():
+    """A request that cannot fit even in the empty pool is rejected (the loop
+    ends) while the others are served."""
+    reqs = make_requests(rate=400.0, duration=0.02, seed=7, n_lo=64, n_hi=96, steps_lo=10,
+                         steps_hi=20)
+    assert len(reqs) >= 2
+    big = type(reqs[0])(len(reqs), reqs[-1].arrival, 600, 10, reqs[0].x0, 3)
+    pool = 2 * job_bytes(PF_STORAGE_F32, 96, 96) + (1 << 20)
+    assert job_bytes(PF_STORAGE_F32, 600, 600) > pool
+    done, stats = serve(reqs + [big], pool, launches_per_graph=3, rows=16)
+    assert stats["rejected"] == 1 and stats["completed"] == len(reqs)
+    assert big not in done
