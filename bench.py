@@ -259,8 +259,9 @@ def batched_section(args):
     for sp in specs:
         s = spec_setup(sp)
         f = spec_fields(sp)
+        # FP32 compositions stored relative to the job's matrix composition
         batch.add(make_desc(s, target, nucleation=True, seed=1000 + sp["index"],
-                            instance=sp["index"]), f)
+                            instance=sp["index"], x_offset=sp["x0"]), f)
         plain_inputs.append((s, f))
     log("batched: %d instances, %.3g cells, setup %.1f s" % (len(specs), cells,
                                                             time.perf_counter() - t_gen))
