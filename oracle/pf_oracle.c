@@ -32,6 +32,14 @@
  *                                      mass conservation, fixed points, Laplacian
  *                                      on quadratics and Fourier modes
  *   Philox4x32-10                     Random123 known-answer vectors
+ *   nucleation uniforms               word order, shift and (0,1] offset computed
+ *                                     exactly from the known-answer words
+ *   diffusion matrix, mobility L      golden D_* at (0.30, 0.02); L against the
+ *                                     closed form on those values, 1/w^2 scaling
+ *                                     and the survey's 4.0015e-8
+ *   field summary                     uniform interiors (exact p(0), p(1/2), p(1)),
+ *                                     NaN / 1e300 ghost rings, dyadic fields with
+ *                                     hand-computed means
  *
  * Array layout: the reference's, (nx = Nx+2) x (ny = Ny+2) row-major FP64 with a
  * one-cell ghost ring, idx = nx*y + x.  Interior cells are 1..nx-2 by 1..ny-2.

@@ -368,6 +368,123 @@ def test_uniforms_in_unit_interval(oracle):
     assert abs(u.mean() - 0.5) < 0.01
 
 
+# Random123 known answers as (counter words, key words, output words).
+_PHILOX_KAT = [
+    ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+]
+
+
+def test_uniforms_bit_mapping_from_known_answers(oracle):
+    """The nucleation stream's word layout and 53-bit mapping, pinned on the
+    Philox known answers: counter = (cell, instance, call lo, call hi), key =
+    (seed lo, seed hi); u1 = ((w1:w0) >> 11) + 1) / 2^53 from words 0-1 and u2
+    from words 2-3, so u lies in (0, 1].  Computed here with exact integer and
+    rational arithmetic from the published output words."""
+    from fractions import Fraction
+    for ctr, key, w in _PHILOX_KAT:
+        cell, instance = ctr[0], ctr[1]
+        call = ctr[2] | (ctr[3] << 32)
+        seed = key[0] | (key[1] << 32)
+        u1, u2 = oracle.nuc_uniforms(seed, call, instance, cell)
+        e1 = Fraction((((w[1] << 32) | w[0]) >> 11) + 1, 1 << 53)
+        e2 = Fraction((((w[3] << 32) | w[2]) >> 11) + 1, 1 << 53)
+        # every value k/2^53 with 1 <= k <= 2^53 is a double: the mapping is exact
+        assert Fraction(u1) == e1
+        assert Fraction(u2) == e2
+    # the ends of the interval: the smallest value is 2^-53, the largest 1
+    assert oracle.nuc_uniforms(0, 0, 0, 0)[0] > 0.0
+
+
+def test_diffusion_matrix_and_mobility_pinned(oracle, thermo_golden):
+    """set_diffusion_matrix (src/alloy625.cpp:80-109) at (0.30, 0.02): D is the
+    reference's compiled D_* at that composition, and the phase mobility is
+    L = MobStab |(D_CrCr, D_NbNb)| / (w^2 RT / Vm) with MobStab = 1 and
+    w = 2.5 nm (src/parameters.h:19-22), the same for both phases."""
+    G = thermo_golden
+    assert tuple(G["comp_in"][0]) == (0.30, 0.02)
+    w = 2.5e-9
+    DCr, DNb, L = oracle.set_diffusion_matrix(0.30, 0.02, w)
+    assert DCr[0] == pytest.approx(G["D_CrCr"][0], rel=1e-14)
+    assert DCr[1] == pytest.approx(G["D_CrNb"][0], rel=1e-14)
+    assert DNb[0] == pytest.approx(G["D_NbCr"][0], rel=1e-14)
+    assert DNb[1] == pytest.approx(G["D_NbNb"][0], rel=1e-14)
+    # the survey's evaluation of the same expression: L = 4.0015e-8 m^2/(N s)
+    assert L[0] == pytest.approx(4.0015e-8, rel=1e-4)
+    assert L[1] == L[0]
+    # the norm is over the diagonal (D_CrCr, D_NbNb), not a row or column: the
+    # off-diagonal entries are ~10x D_CrCr, so mixing them in moves L by > 0.1%
+    RT, Vm = G["const_RT"][0], G["const_Vm"][0]
+    diag = math.hypot(G["D_CrCr"][0], G["D_NbNb"][0])
+    assert L[0] == pytest.approx(diag / (w * w * RT / Vm), rel=1e-14)
+    for wrong in (math.hypot(G["D_CrCr"][0], G["D_CrNb"][0]),
+                  math.hypot(G["D_NbCr"][0], G["D_NbNb"][0])):
+        assert abs(wrong / (w * w * RT / Vm) / L[0] - 1.0) > 1e-3
+    # L scales as 1/w^2 (the interface width enters squared)
+    _, _, L2 = oracle.set_diffusion_matrix(0.30, 0.02, 2.0 * w)
+    assert L2[0] == pytest.approx(L[0] / 4.0, rel=1e-15)
+    # the transformation-limited step of src/main.cpp:260 with kappa of
+    # src/parameters.h:43: dx^2 / (4 L kappa) = 8.811e-4 s at dx = 0.25 nm
+    kappa = 3.0 * 0.13 * w / 2.2
+    assert (0.25e-9) ** 2 / (4.0 * L[0] * kappa) == pytest.approx(8.811e-4, rel=1e-3)
+
+
+def _summary_fields(Nx, Ny, ghost):
+    return [np.full((Ny + 2, Nx + 2), ghost) for _ in range(4)]
+
+
+def test_summarize_uniform_fields_exact(oracle):
+    """summarize_fields (src/alloy625.cpp:811-863) on uniform interiors gives
+    exactly (x_Cr, x_Nb, 1 - p(phi_d) - p(phi_l), p(phi_d), p(phi_l)); p(0) = 0,
+    p(1/2) = 1/2, p(1) = 1 are exact in binary.  The ghost ring holds NaN or
+    1e300, so any ghost cell in the sum shows."""
+    Nx, Ny = 12, 9
+    cases = [(0.0, 0.0, (1.0, 0.0, 0.0)), (0.5, 0.0, (0.5, 0.5, 0.0)),
+             (1.0, 0.0, (0.0, 1.0, 0.0)), (0.0, 1.0, (0.0, 0.0, 1.0)),
+             (0.5, 0.5, (0.0, 0.5, 0.5))]
+    for ghost in (np.nan, 1e300):
+        for pd, pl, (gam, dl, lv) in cases:
+            f = _summary_fields(Nx, Ny, ghost)
+            for a, v in zip(f, (0.25, 0.0625, pd, pl)):
+                a[1:-1, 1:-1] = v
+            out = oracle.summarize(f)
+            assert out == [0.25, 0.0625, gam, dl, lv], (ghost, pd, pl, out)
+
+
+def test_summarize_dyadic_sum(oracle):
+    """A field of dyadic values whose sums are exact in binary: the means are
+    the hand-computed ones.  x_Cr = k/64 over a 4 x 3 interior in row order
+    (k = 0..11) sums to 66/64, so its mean is 66/64/12 = 11/128; x_Nb is 1/8 on
+    the first row and 0 elsewhere (mean 1/24 up to rounding of the division);
+    phi_d = 1 on two cells, 1/2 on two (p = 1, 1/2), so delta = 3/12 = 1/4;
+    phi_l = 1 on one cell, so Laves = 1/12 and gamma = 1 - 1/4 - 1/12 = 2/3."""
+    Nx, Ny = 4, 3
+    f = _summary_fields(Nx, Ny, np.nan)
+    f[0][1:-1, 1:-1] = (np.arange(12, dtype=np.float64) / 64.0).reshape(3, 4)
+    f[1][1:-1, 1:-1] = 0.0
+    f[1][1, 1:-1] = 0.125
+    f[2][1:-1, 1:-1] = 0.0
+    f[2][1, 1] = f[2][2, 3] = 1.0
+    f[2][3, 2] = f[2][3, 4] = 0.5
+    f[3][1:-1, 1:-1] = 0.0
+    f[3][2, 1] = 1.0
+    out = oracle.summarize(f)
+    assert out[0] == 11.0 / 128.0
+    assert out[1] == pytest.approx(1.0 / 24.0, rel=1e-16)
+    assert out[3] == 0.25
+    assert out[4] == pytest.approx(1.0 / 12.0, rel=1e-16)
+    assert out[2] == pytest.approx(2.0 / 3.0, rel=1e-16)
+    # the phases are the interpolated fractions, not the raw order parameter:
+    # phi_d = 1/4 gives p = 0.103515625 (exact in binary), not 1/4
+    f[2][1:-1, 1:-1] = 0.25
+    f[3][1:-1, 1:-1] = 0.0
+    out = oracle.summarize(f)
+    assert out[3] == 0.103515625
+    assert out[2] == 1.0 - 0.103515625
+
+
 def synthetic_nucleation_case(oracle, nx=40, ny=34, seed=11):
     """Fields and nucleation arguments for which P lies inside (0, 1) on most
     cells (at the benchmark's own parameters P rounds to 0).  The domain is
