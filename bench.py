@@ -9,31 +9,51 @@ A "step" is one explicit Euler step over the whole grid; steps run three per
 launch (the temporally blocked kernel) except around nucleation steps, which
 run one per launch with the nucleation pass fused.  value = interior cells x
 timed steps / time of the timed loop (HIP events), fields resident in HBM.
+
+Clock state: the GPU's clock takes ~50 launches of sustained load to settle
+(the first launches of a cold run are up to 30% slower), so before the warm-up
+steps the three-step kernel runs --settle launches on scratch copies of the
+fields; the timed loop and the kernel leg then run in the same settled state,
+and the line fails (no output, exit 3) if the kernel leg's time per step
+exceeds the timed loop's.
+
 roofline: the dominant kernel is the three-step kernel; its algorithmic bytes
 per launch (FP64: 4 fields read + 4 written = 64 B per cell, for three
 cell-updates) over its average launch duration (HIP events on the launch
-stream); the two-step and one-step kernels are reported beside it.  traffic:
-HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs under
-profiles/ (scaled by a copy kernel).
-cpu_baseline: the FP64 CPU oracle (OpenMP, all host threads) on a bounded
-sample of the same workload.
+stream, --kernel-reps back-to-back launches right after the timed loop); the
+two-step and one-step kernels are reported beside it.  traffic: HBM bytes per
+launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs under profiles/ (scaled
+by a copy kernel).
+cpu_baseline: the FP64 CPU oracle built -O3 -march=native on the host
+(oracle/cpu_bench.py, a child process, OpenMP on every physical core with
+OMP_PROC_BIND=close) on a bounded sample of the same workload; beside it one
+core, and the 256^2 plain case (1000 steps) in full.
 
 Also reported (same JSON line):
-  dropin   -- the reference driver's per-step call sequence through the C++
-              drop-in API (tests/cpp/replay_main) on the same grid (no-flux,
-              nucleation on, no I/O), grouped three steps per launch and one.
-  batched  -- 512 independent simulations, square sides log-uniform in
-              128..1024, FP32 storage with FP64 arithmetic and reductions, one
-              persistent launch per one, two or three steps replayed from a
-              hipGraph; tile, steps per launch and launches per graph from the
-              autotuner; against loops of plain FP64 launches (three-, two-
-              and one-step) over the same instances; phase-fraction error
-              against FP64 replays of sampled instances.
-  serving  -- Poisson sweep requests (side log-uniform 128..2048, 500..20000
-              steps, random composition and nucleation seed) with continuous
-              batching under an HBM budget; throughput and p50/p99 latency.
+  dropin    -- the reference driver's per-step call sequence through the C++
+               drop-in API (tests/cpp/replay_main) on the same grid (no-flux,
+               nucleation on, no I/O), grouped three steps per launch and one.
+  batched   -- 512 independent simulations, square sides log-uniform in
+               128..1024, FP32 storage with FP64 arithmetic and reductions, one
+               persistent launch per one, two or three steps replayed from a
+               hipGraph; tile, steps per launch and launches per graph from the
+               autotuner; against loops of plain FP64 launches (three-, two-
+               and one-step) over the same instances; phase-fraction error
+               against FP64 replays of sampled instances.
+  precision -- 8192^2, the headline's initialisation, FP32 storage (with and
+               without a composition offset) against FP64, 20,000 steps: max
+               field differences, phase fractions and mass drift at 1k/10k/20k
+               steps; and the same grid with seeded delta and Laves particles
+               (FP64 against FP32 with the offset) so that the phase-fraction
+               error is measured where the fractions are not zero.
+  serving   -- Poisson sweep requests (side log-uniform 128..2048, 500..20000
+               steps, random composition and nucleation seed) with continuous
+               batching under an HBM budget, at several arrival rates:
+               throughput, p50/p99 latency and the device-busy share; and the
+               shortest-remaining-work order against FIFO at the highest rate.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--form plain|precision]
+    python bench.py --trace-summary <rocprofv3 kernel_trace.csv>
 """
 import argparse
 import ctypes
@@ -80,52 +100,77 @@ def load_valu_issue(kernel):
 
 
 def host_cpu():
-    """Model name, logical CPUs visible and socket count of the host."""
-    model, sockets = "unknown", set()
+    """Model name, logical CPUs this process may run on, the physical cores
+    among them (distinct (package, core) pairs from /sys) and the sockets."""
+    model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 k, _, v = line.partition(":")
-                k = k.strip()
-                if k == "model name" and model == "unknown":
+                if k.strip() == "model name":
                     model = v.strip()
-                elif k == "physical id":
-                    sockets.add(v.strip())
+                    break
     except OSError:
         pass
-    return {"model": model, "logical_cpus": os.cpu_count(), "sockets": len(sockets) or None}
-
-
-def cpu_baseline(N, steps, warmup=1):
-    """Time the FP64 oracle (all host threads via OpenMP) on N x N for `steps`."""
-    import oracle as O
-    from pfpa import model as M
-    s = M.make_setup(N, N, bc=1)
-    f0 = M.init_enriched_band(N, N, 1, seed=42)
-    P = O.make_params(s.dt, s.DCr, s.DNb, s.kappa, s.omega, s.Lmob, s.alpha, s.mask)
-    O.run(f0, P, 1, warmup)
-    t0 = time.perf_counter()
-    O.run(f0, P, 1, steps)
-    el = time.perf_counter() - t0
-    return N * N * steps / el, O.num_threads(), el
-
-
-def cpu_baseline_single(N, steps):
-    """The same on one thread (a shorter sample); restores the thread count."""
-    import oracle as O
-    from pfpa import model as M
-    s = M.make_setup(N, N, bc=1)
-    f0 = M.init_enriched_band(N, N, 1, seed=42)
-    P = O.make_params(s.dt, s.DCr, s.DNb, s.kappa, s.omega, s.Lmob, s.alpha, s.mask)
-    threads = O.num_threads()
-    O.set_threads(1)
     try:
-        t0 = time.perf_counter()
-        O.run(f0, P, 1, steps)
-        el = time.perf_counter() - t0
-    finally:
-        O.set_threads(threads)
-    return N * N * steps / el, el
+        cpus = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        cpus = list(range(os.cpu_count() or 1))
+    cores, sockets = set(), set()
+    for c in cpus:
+        base = "/sys/devices/system/cpu/cpu%d/topology/" % c
+        try:
+            with open(base + "physical_package_id") as f:
+                pkg = f.read().strip()
+            with open(base + "core_id") as f:
+                core = f.read().strip()
+        except OSError:
+            continue
+        cores.add((pkg, core))
+        sockets.add(pkg)
+    return {"model": model, "logical_cpus": len(cpus), "physical_cores": len(cores) or None,
+            "sockets": len(sockets) or None, "os_cpu_count": os.cpu_count()}
+
+
+def cpu_baseline(args):
+    """The FP64 CPU oracle built -O3 -march=native for this host, timed in a
+    child process (oracle/cpu_bench.py) on every physical core this process
+    may use (OMP_PROC_BIND=close) and on one core."""
+    import subprocess
+    cpu = host_cpu()
+    # every physical core, within the 16-CPU share a one-GPU box gives a job
+    threads = min(16, cpu["physical_cores"] or cpu["logical_cpus"])
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+               OMP_PLACES="cores")
+    cmd = [sys.executable, os.path.join(ROOT, "oracle", "cpu_bench.py"),
+           "--k1-size", str(args.size), "--k1-steps", str(args.cpu_steps),
+           "--k1-one-steps", str(args.cpu_one_steps), "--k0-steps", "1000"]
+    r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError("CPU baseline failed: " + r.stderr[-2000:])
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    k1, k1o, k0 = d["k1"], d["k1_one"], d["k0"]
+    return {
+        "value": k1["cell_updates_per_s"], "unit": "cell-updates/s", "cores": k1["threads"],
+        "kind": "port",
+        "sample": "%d steps of the %dx%d FP64 plain step (same init, periodic, nucleation "
+                  "off) after 1 warm-up step, %.1f s" % (k1["steps"], args.size, args.size,
+                                                          k1["seconds"]),
+        "build": d["build"] + " (timed); the checker is built -O2 -ffp-contract=off",
+        "omp": {"OMP_NUM_THREADS": threads, "OMP_PROC_BIND": "close", "OMP_PLACES": "cores"},
+        "host_cpu": cpu,
+        "one_core": {"value": k1o["cell_updates_per_s"], "cores": 1,
+                     "sample": "%d steps of the same %dx%d step, %.1f s"
+                               % (k1o["steps"], args.size, args.size, k1o["seconds"])},
+        "k0_256_1000_steps": {
+            "grid": [256, 256], "steps": k0["steps"], "bc": "periodic",
+            "nucleation": False, "dt": k0["dt"],
+            "all_cores": {"value": k0["all_threads"]["cell_updates_per_s"],
+                          "cores": k0["all_threads"]["threads"],
+                          "seconds": k0["all_threads"]["seconds"]},
+            "one_core": {"value": k0["one_thread"]["cell_updates_per_s"], "cores": 1,
+                         "seconds": k0["one_thread"]["seconds"]}},
+    }
 
 
 def _ptr(t):
@@ -154,48 +199,68 @@ def plain_section(args, dtype_name="f64"):
 
     N = args.size
     dtype = torch.float64 if dtype_name == "f64" else torch.float32
+    f64 = dtype == torch.float64
     s = M.make_setup(N, N, bc=1)
     f0 = M.init_enriched_band(N, N, 1, seed=42)
     sim = Simulation(s, f0, dtype=dtype, nucleation=True, seed=42)
+    del f0
     stream = sim.stream
+    P = sim.P
+    st = ctypes.c_void_p(stream.cuda_stream)
+    # scratch copies of the fields: the clock-settling launches and the
+    # kernel leg run on these, so the simulation itself advances exactly
+    # warmup + steps steps
+    with torch.cuda.stream(stream):
+        a = [x.clone() for x in sim.a]
+        b = [torch.empty_like(x) for x in a]
+    fn = {3: lib().pf_step3_f64 if f64 else lib().pf_step3_f32,
+          2: lib().pf_step2_f64 if f64 else lib().pf_step2_f32,
+          1: lib().pf_step_f64 if f64 else lib().pf_step_f32}
+
+    def launch(k, n):
+        for i in range(n):
+            x, y = (a, b) if i % 2 == 0 else (b, a)
+            args_ = [sim.ctx, s.nx, s.ny, s.bc, ctypes.byref(P), *[_ptr(t) for t in x],
+                     *[_ptr(t) for t in y]] + ([None] if k == 1 else []) + [st]
+            check(fn[k](*args_), "step")
+
+    def time_kernel(k, reps):
+        k0 = torch.cuda.Event(enable_timing=True)
+        k1 = torch.cuda.Event(enable_timing=True)
+        k0.record(stream)
+        launch(k, reps)
+        k1.record(stream)
+        k1.synchronize()
+        return k0.elapsed_time(k1) / 1e3 / reps
+
+    # settle the clock on the scratch copies, then warm up and time the
+    # simulation's own steps
+    settle_s = time_kernel(3, args.settle) if args.settle > 0 else 0.0
     sim.step(args.warmup)
-    torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     sim.step(args.steps)
     e1.record(stream)
-    torch.cuda.synchronize()
+    e1.synchronize()
     ms = e0.elapsed_time(e1)
     cells = N * N
     value = cells * args.steps / (ms / 1e3)
 
-    # The kernels alone, back to back on the sim's stream: the three-step kernel
-    # (the dominant one), the two-step kernel (what the schedule leaves between
-    # nucleation steps) and the one-step kernel (nucleation steps; the drop-in
-    # API with grouping off).
-    P = sim.P
-    a, b = sim.a, sim.b
-    st = ctypes.c_void_p(stream.cuda_stream)
-    f64 = dtype == torch.float64
-    reps = max(20, min(200, args.steps))
-
-    def time_kernel(fn, nuc_arg):
-        k0 = torch.cuda.Event(enable_timing=True)
-        k1 = torch.cuda.Event(enable_timing=True)
-        k0.record(stream)
-        for i in range(reps):
-            x, y = (a, b) if i % 2 == 0 else (b, a)
-            args_ = [sim.ctx, s.nx, s.ny, s.bc, ctypes.byref(P), *[_ptr(t) for t in x],
-                     *[_ptr(t) for t in y]] + ([None] if nuc_arg else []) + [st]
-            check(fn(*args_), "step")
-        k1.record(stream)
-        torch.cuda.synchronize()
-        return k0.elapsed_time(k1) / 1e3 / reps
-
-    kern3_s = time_kernel(lib().pf_step3_f64 if f64 else lib().pf_step3_f32, False)
-    kern2_s = time_kernel(lib().pf_step2_f64 if f64 else lib().pf_step2_f32, False)
-    kern1_s = time_kernel(lib().pf_step_f64 if f64 else lib().pf_step_f32, True)
+    # The kernels alone, back to back on the sim's stream right after the
+    # timed loop (same clock state): the three-step kernel (the dominant
+    # one), the two-step kernel (what the schedule leaves between nucleation
+    # steps) and the one-step kernel (nucleation steps; the drop-in API with
+    # grouping off).
+    kern3_s = time_kernel(3, args.kernel_reps)
+    kern2_s = time_kernel(2, args.kernel_reps)
+    kern1_s = time_kernel(1, args.kernel_reps)
+    if kern3_s / 3 * 1e3 > ms / args.steps:
+        log("bench.py: the three-step kernel's time per step (%.2f us, %d back-to-back "
+            "launches) exceeds the timed loop's (%.2f us): the two measurements are not in "
+            "the same clock state" % (kern3_s / 3 * 1e6, args.kernel_reps,
+                                      ms / args.steps * 1e3))
+        sys.exit(3)
     bpc = 64 if f64 else 32
     alg_bytes = bpc * cells          # 4 fields read + 4 written per launch
     tag = "fp64" if f64 else "fp32"
@@ -213,6 +278,15 @@ def plain_section(args, dtype_name="f64"):
     return {
         "value": value, "ms_per_step": ms / args.steps, "N": N, "nuc_every": nuc_every,
         "dt": s.dt,
+        "timing": {
+            "clock_settle": "%d back-to-back three-step launches on scratch copies of the "
+                            "fields before the warm-up (%.1f us each)"
+                            % (args.settle, settle_s * 1e6),
+            "kernel_leg": "%d back-to-back launches of each kernel on the scratch copies right "
+                          "after the timed loop" % args.kernel_reps,
+            "three_step_kernel_us_per_step": kern3_s / 3 * 1e6,
+            "timed_loop_us_per_step": ms / args.steps * 1e3,
+        },
         "roofline": {
             "bound": "hbm", "achieved": alg_bytes / kern3_s / 1e9, "peak": HBM_PEAK / 1e9,
             "unit": "GB/s", "frac": alg_bytes / kern3_s / HBM_PEAK,
@@ -395,90 +469,156 @@ def dropin_section(args):
             "nucleation_interval": nuc, **out}
 
 
-def serving_section(args):
+def serving_section(args, capacity=None):
+    """The serving loop at each rate of --serve-rates (FIFO), and the
+    shortest-remaining-work order at the highest rate, on the same seeded
+    request streams.  capacity: the batched form's cell-updates/s, for the
+    share of it each rate uses."""
     from pfpa.serve import fp64_replay_error, make_requests, serve
-    reqs = make_requests(args.serve_rate, args.serve_duration, seed=1)
-    done = [0]
 
-    def progress(r):   # one line per 100 completions (long runs stay visibly alive)
-        done[0] += 1
-        if done[0] % 100 == 0:
-            log("serving: %d of %d requests done" % (done[0], len(reqs)))
+    def one(rate, order):
+        reqs = make_requests(rate, args.serve_duration, seed=1)
+        done = [0]
 
-    finished, stats = serve(reqs, pool_bytes=args.serve_pool_gb << 30,
-                            launches_per_graph=args.serve_lpg, rows=args.serve_rows,
-                            depth=3, launch_steps=3, log=progress)
-    sample = sorted(finished, key=lambda r: r.n * r.n * r.steps)[:3]
-    err, _ = fp64_replay_error(finished, sample)
-    stats["arrival_rate_per_s"] = args.serve_rate
-    stats["arrival_window_s"] = args.serve_duration
-    stats["phase_fraction_max_abs_error_vs_fp64"] = err
-    stats["fp64_replayed"] = len(sample)
-    return stats
+        def progress(r):   # one line per 100 completions (long runs stay visibly alive)
+            done[0] += 1
+            if done[0] % 100 == 0:
+                log("serving %s %.1f/s: %d of %d requests done" % (order, rate, done[0],
+                                                                   len(reqs)))
+
+        finished, stats = serve(reqs, pool_bytes=args.serve_pool_gb << 30,
+                                launches_per_graph=args.serve_lpg, rows=args.serve_rows,
+                                depth=3, launch_steps=3, log=progress, order=order)
+        sample = sorted(finished, key=lambda r: r.n * r.n * r.steps)[:3]
+        err, _ = fp64_replay_error(finished, sample)
+        stats["arrival_rate_per_s"] = rate
+        stats["arrival_window_s"] = args.serve_duration
+        stats["phase_fraction_max_abs_error_vs_fp64"] = err
+        stats["fp64_replayed"] = len(sample)
+        if capacity:
+            stats["throughput_share_of_batched_capacity"] = \
+                stats["throughput_cell_updates_per_s"] / capacity
+        log("serving:", json.dumps({k: v for k, v in stats.items()
+                                    if k != "latency_by_size"}))
+        return stats
+
+    rates = sorted(args.serve_rates)
+    sweep = [one(r, "fifo") for r in rates]
+    out = {"rates": sweep, "batched_capacity_cell_updates_per_s": capacity,
+           "pool_bytes": args.serve_pool_gb << 30,
+           "requests": "Poisson arrivals over %.0f s; side log-uniform 128..2048, steps "
+                       "uniform 500..20000, composition uniform in the matrix or enriched "
+                       "range, random nucleation seed (seed 1 at every rate)"
+                       % args.serve_duration}
+    if not args.no_srw:
+        out["srw_at_highest_rate"] = one(rates[-1], "srw")
+    return out
+
+
+def _precision_run(args, s, f0, forms, label):
+    """Step the forms {name: (dtype, storage_offset)} of one workload side by
+    side to each mark; rows of max |dx| against FP64, phase fractions and
+    each form's own mass drift; rates from HIP events around the stepping."""
+    import torch
+
+    from pfpa.sim import Simulation
+    N = s.N[0]
+    sims = {n: Simulation(s, f0, dtype=dt, nucleation=True, seed=42, storage_offset=off)
+            for n, (dt, off) in forms.items()}
+    m0 = {n: sim.summarize() for n, sim in sims.items()}
+    ref = sims["fp64"]
+    secs = {n: 0.0 for n in sims}
+    rows = []
+    done = 0
+    for mark in args.precision_marks:
+        for n, sim in sims.items():
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(sim.stream)
+            sim.step(mark - done)
+            e1.record(sim.stream)
+            e1.synchronize()
+            secs[n] += e0.elapsed_time(e1) / 1e3
+        done = mark
+        summ = {n: sim.summarize() for n, sim in sims.items()}
+        row = {"step": mark}
+        for n, sim in sims.items():
+            if n != "fp64":
+                for k, c in ((0, "Cr"), (1, "Nb")):
+                    row["max_abs_dx_%s_%s" % (c, n)] = float(
+                        (ref.a[k] - (sim.a[k].double() + sim.xoff[k])).abs().max())
+        for n in sims:
+            row["delta_frac_" + n] = summ[n][3]
+            row["laves_frac_" + n] = summ[n][4]
+            if n != "fp64":
+                row["phase_frac_abs_err_" + n] = max(abs(summ[n][3] - summ["fp64"][3]),
+                                                     abs(summ[n][4] - summ["fp64"][4]))
+        for n in sims:
+            row["mass_drift_Cr_" + n] = summ[n][0] - m0[n][0]
+            row["mass_drift_Nb_" + n] = summ[n][1] - m0[n][1]
+        rows.append(row)
+        log("precision %s" % label, json.dumps(row))
+    offsets = {n: list(sim.xoff) for n, sim in sims.items() if sim.xoff != (0.0, 0.0)}
+    for sim in sims.values():
+        sim.close()
+    rates = {n: N * N * done / secs[n] for n in sims}
+    return {"workload": label, "rows": rows, "rates_cell_updates_per_s": rates,
+            "offsets": offsets, "steps": done}
 
 
 def precision_section(args):
-    """FP32 storage, without and with a composition offset (the median
-    composition), vs FP64 on 8192^2 (same init as the plain case): max field
-    differences, phase fractions and mass drift at 1k/10k/20k steps."""
+    """8192^2 with the headline's initialisation: FP32 storage without and
+    with a composition offset (the median composition) against FP64; then
+    the same initialisation with seeded delta and Laves particles, FP32 with
+    the offset against FP64, so that the phase-fraction error is measured
+    where the fractions are not zero."""
     import torch
 
     from pfpa import model as M
-    from pfpa.sim import Simulation
-    N = args.size
+    N = args.precision_size
     s = M.make_setup(N, N, bc=1)
     f0 = M.init_enriched_band(N, N, 1, seed=42)
-    s64 = Simulation(s, f0, dtype=torch.float64, nucleation=True, seed=42)
-    s32 = Simulation(s, f0, dtype=torch.float32, nucleation=True, seed=42)
-    s32o = Simulation(s, f0, dtype=torch.float32, nucleation=True, seed=42,
-                      storage_offset="median")
+    plain = _precision_run(args, s, f0, {
+        "fp64": (torch.float64, None), "fp32": (torch.float32, None),
+        "fp32_offset": (torch.float32, "median")},
+        "%dx%d, the headline's initialisation (no precipitates), periodic, nucleation on"
+        % (N, N))
+    n = args.precision_particles
+    M.add_particles(f0, s.dx, 1, seed=7, n_del=n, n_lav=n, radius=5e-9)
+    parts = _precision_run(args, s, f0, {
+        "fp64": (torch.float64, None), "fp32_offset": (torch.float32, "median")},
+        "%dx%d, the same initialisation with %d delta + %d Laves particles (R = 5 nm, "
+        "seed 7), periodic, nucleation on" % (N, N, n, n))
     del f0
-    m0 = s64.summarize()
-    rows = []
-    done = 0
-    secs = {64: 0.0, 32: 0.0, "32o": 0.0}   # GPU time of each form's stepping (HIP events)
+    return {"grid": [N, N], "dtypes": "FP32 storage, FP64 arithmetic (KKS solve included) and "
+                                      "compensated FP64 reductions, against FP64 storage",
+            "no_precipitates": plain, "with_precipitates": parts}
 
-    def timed_steps(sim, n, key):
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(sim.stream)
-        sim.step(n)
-        e1.record(sim.stream)
-        e1.synchronize()
-        secs[key] += e0.elapsed_time(e1) / 1e3
 
-    for mark in args.precision_marks:
-        timed_steps(s64, mark - done, 64)
-        timed_steps(s32, mark - done, 32)
-        timed_steps(s32o, mark - done, "32o")
-        done = mark
-
-        def max_dx(sim, k):
-            return float((s64.a[k] - (sim.a[k].double() + sim.xoff[k])).abs().max())
-
-        r64, r32, r32o = s64.summarize(), s32.summarize(), s32o.summarize()
-        rows.append({"step": mark, "max_abs_dx_Cr": max_dx(s32, 0),
-                     "max_abs_dx_Nb": max_dx(s32, 1),
-                     "max_abs_dx_Cr_offset": max_dx(s32o, 0),
-                     "max_abs_dx_Nb_offset": max_dx(s32o, 1),
-                     "delta_frac_64": r64[3], "delta_frac_32": r32[3],
-                     "delta_frac_32_offset": r32o[3],
-                     "laves_frac_64": r64[4], "laves_frac_32": r32[4],
-                     "laves_frac_32_offset": r32o[4],
-                     "mass_drift_Cr_64": r64[0] - m0[0], "mass_drift_Cr_32": r32[0] - m0[0],
-                     "mass_drift_Cr_32_offset": r32o[0] - m0[0],
-                     "mass_drift_Nb_64": r64[1] - m0[1], "mass_drift_Nb_32": r32[1] - m0[1],
-                     "mass_drift_Nb_32_offset": r32o[1] - m0[1]})
-        log("precision", json.dumps(rows[-1]))
-    offset = list(s32o.xoff)
-    s64.close()
-    s32.close()
-    s32o.close()
-    rates = {"fp64": N * N * done / secs[64], "fp32_storage": N * N * done / secs[32],
-             "fp32_storage_offset": N * N * done / secs["32o"], "offset": offset,
-             "unit": "cell-updates/s", "steps": done}
-    log("precision rates", json.dumps(rates))
-    return rows, rates
+def trace_summary(path, last=200):
+    """Per (kernel, grid) launch counts and durations from a rocprofv3
+    --kernel-trace CSV: all launches, and the last `last` launches of each
+    group (bench.py's kernel leg is the last run of each step kernel at the
+    headline's grid).  Prints a table."""
+    import csv
+    groups = {}
+    with open(path) as f:
+        rd = csv.DictReader(f)
+        cols = rd.fieldnames or []
+        gcols = [c for c in cols if c.startswith("Grid_Size")]
+        for row in rd:
+            key = (row["Kernel_Name"],) + tuple(row[c] for c in gcols)
+            dur = (int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) / 1e3
+            groups.setdefault(key, []).append((int(row["Start_Timestamp"]), dur))
+    print("%-60s %-18s %6s %9s %9s %9s %12s" % ("kernel", "grid", "calls", "avg_us", "min_us",
+                                                 "max_us", "last%d_avg_us" % last))
+    for key, v in sorted(groups.items(), key=lambda kv: -sum(d for _, d in kv[1])):
+        v.sort()
+        d = [x for _, x in v]
+        tail = d[-last:]
+        print("%-60s %-18s %6d %9.1f %9.1f %9.1f %12.1f" % (
+            key[0][:60], "x".join(key[1:]), len(d), sum(d) / len(d), min(d), max(d),
+            sum(tail) / len(tail)))
 
 
 def main():
@@ -486,37 +626,54 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     # the 4096^2 plain workload's run length (about 0.9 s of GPU time)
     ap.add_argument("--steps", type=int, default=10000)
-    # the GPU's clock takes about 30 ms of sustained load to settle (the first
-    # launches of a cold run are up to 30% slower): 600 steps are ~55 ms
     ap.add_argument("--warmup", type=int, default=600)
+    # the GPU's clock takes ~50 launches (~30 ms) of sustained load to settle:
+    # three-step launches on scratch copies of the fields before the warm-up
+    ap.add_argument("--settle", type=int, default=200)
+    ap.add_argument("--kernel-reps", type=int, default=200,
+                    help="back-to-back launches of each step kernel for the roofline")
     ap.add_argument("--form", choices=["plain", "precision"], default="plain")
     ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-one-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-only", action="store_true")
-    ap.add_argument("--only", choices=["dropin", "batched", "serving"], default=None,
-                    help="run one of the other sections alone (no headline line)")
+    ap.add_argument("--only", choices=["dropin", "batched", "serving", "precision"],
+                    default=None, help="run one of the other sections alone (no headline line)")
     ap.add_argument("--dropin-steps", type=int, default=900,
                     help="steps of the C++ drop-in replay (the driver's call sequence)")
     ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=5000)
     ap.add_argument("--loop-steps", type=int, default=20,
                     help="launches of each plain loop (two-step and one-step)")
-    ap.add_argument("--serve-rate", type=float, default=6.0)
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[6.0, 16.0, 24.0])
     ap.add_argument("--serve-duration", type=float, default=20.0)
     ap.add_argument("--serve-pool-gb", type=int, default=4)
     ap.add_argument("--serve-lpg", type=int, default=17,
                     help="step launches per captured graph in the serving loop")
     ap.add_argument("--serve-rows", type=int, default=16,
                     help="rows per strip task of the serving batch")
+    ap.add_argument("--no-srw", action="store_true",
+                    help="skip the shortest-remaining-work serving run")
+    ap.add_argument("--precision-size", type=int, default=8192)
     ap.add_argument("--precision-marks", type=int, nargs="+", default=[1000, 10000, 20000])
+    ap.add_argument("--precision-particles", type=int, default=256,
+                    help="delta and Laves particles (each) of the precision form's second "
+                         "workload")
+    ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the one-, two- and three-step kernels (FP64, FP32) at --size "
                          "a few times each, for rocprofv3 counter runs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain device copy of 4 x 4098^2 FP64 arrays (1.07 GB moved per "
                          "dispatch) to scale rocprofv3 FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--trace-summary", default=None,
+                    help="summarize a rocprofv3 kernel_trace.csv per kernel and grid, and exit")
     args = ap.parse_args()
+
+    if args.trace_summary:
+        trace_summary(args.trace_summary, last=args.kernel_reps)
+        return
 
     if args.gpus != 1:
         print("bench.py: runs on one GPU; --gpus %d is not supported (the y-slab "
@@ -528,7 +685,7 @@ def main():
         n = 4098 * 4098
         src = [torch.ones(n, dtype=torch.float64, device="cuda") for _ in range(4)]
         dst = [torch.empty_like(x) for x in src]
-        for _ in range(5):
+        for _ in range(25):
             for a, b in zip(dst, src):
                 a.copy_(b)
         torch.cuda.synchronize()
@@ -557,25 +714,20 @@ def main():
                                                                "three_step": 7}}))
         return
 
+    if args.form == "precision":
+        args.only = "precision"
     if args.only:
         # one section alone (for profiling), one JSON line
         args.size = args.size or 4096
         fn = {"dropin": dropin_section, "batched": batched_section,
-              "serving": serving_section}[args.only]
+              "serving": serving_section, "precision": precision_section}[args.only]
         print(json.dumps({args.only: fn(args)}))
-        return
-
-    if args.form == "precision":
-        args.size = args.size or 8192
-        rows, rates = precision_section(args)
-        print(json.dumps({"form": "precision", "grid": [args.size, args.size], "rows": rows,
-                          "rates": rates}))
         return
 
     args.size = args.size or 4096
     t_all = time.perf_counter()
     plain = plain_section(args)
-    log("plain: %.4g cell-updates/s, %.1f us/step kernel" %
+    log("plain: %.4g cell-updates/s, %.1f us/launch three-step kernel" %
         (plain["value"], plain["roofline"]["kernel_avg_us"]))
     out = {
         "metric": METRIC,
@@ -599,26 +751,22 @@ def main():
             "parallelism": "single GPU",
         },
         "roofline": plain["roofline"],
+        "timing": plain["timing"],
     }
     if not args.plain_only:
         out["dropin"] = dropin_section(args)
         log("dropin:", json.dumps(out["dropin"]))
         out["batched"] = batched_section(args)
-        log("batched:", json.dumps(out["batched"]))
-        out["serving"] = serving_section(args)
-        log("serving:", json.dumps(out["serving"]))
+        log("batched:", json.dumps({k: v for k, v in out["batched"].items()
+                                    if k != "autotuned"}))
+        if not args.no_precision:
+            out["precision"] = precision_section(args)
+        out["serving"] = serving_section(args, capacity=out["batched"]["value"])
     cpu = None
     if not args.no_cpu:
-        cpu_N = 4096 if args.size >= 4096 else args.size
-        v, cores, el = cpu_baseline(cpu_N, args.cpu_steps)
-        cpu = {"value": v, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-               "sample": "%d steps of the %dx%d FP64 plain step (same init, periodic) after "
-                         "1 warm-up step, %.1f s" % (args.cpu_steps, cpu_N, cpu_N, el),
-               "host_cpu": host_cpu()}
-        v1, el1 = cpu_baseline_single(cpu_N, 2)
-        cpu["single_thread"] = {"value": v1, "unit": "cell-updates/s", "cores": 1,
-                                "sample": "2 steps of the same %dx%d step, %.1f s"
-                                          % (cpu_N, cpu_N, el1)}
+        cpu = cpu_baseline(args)
+        cpu["gpu_over_cpu_same_workload"] = plain["value"] / cpu["value"]
+        log("cpu:", json.dumps(cpu))
     out["cpu_baseline"] = cpu
     out["bench_wall_s"] = time.perf_counter() - t_all
     print(json.dumps(out))

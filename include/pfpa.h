@@ -200,6 +200,33 @@ int pf_step_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream
 int pf_step2_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream);
 int pf_step3_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream);
 
+/* Variable-mobility Cahn-Hilliard divergence: the face-averaged
+ * discrete_laplacian sums of chemical_convolution_Cr_kernel /
+ * chemical_convolution_Nb_kernel (src/discretization.cu:605-778; in the
+ * reference they compile but their launch is commented out at :809-835, and
+ * the mobility kernels that would fill the arrays, :175-358, call d_M_*
+ * functions no file defines, so the mobilities are caller inputs).  All
+ * arrays are DEVICE nx x ny FP64 with the ghost ring, filled by the caller
+ * (no-flux copy or periodic wrap).  For component i (Cr, Nb) and every cell,
+ * ghost ring included:
+ *   div_i = sum_{t<6} [D_x + D_y](mob_i[t], conc[t]) + sum_{k<2} D_x(mob_i[6+k], phi_k),
+ *   D_x(M, c) = ((M[x+1] + M[x]) (c[x+1] - c[x]) - (M[x] + M[x-1]) (c[x] - c[x-1])) / (2 dx^2)
+ * (D_y likewise with dy), where the neighbour beyond the array edge is the
+ * cell itself (the reference's index clamp).  The phase-field terms are
+ * differenced in x only, as in the reference (:688-695, :769-776).
+ * div_Cr / div_Nb must not overlap any input. */
+typedef struct pf_mobility_fields {
+	const double* conc[6];    /* x_gam_Cr, x_gam_Nb, x_del_Cr, x_del_Nb, x_lav_Cr, x_lav_Nb */
+	const double* mob_Cr[8];  /* gam_CrCr, gam_CrNb, del_CrCr, del_CrNb, lav_CrCr, lav_CrNb,
+	                             phi_del_Cr, phi_lav_Cr */
+	const double* mob_Nb[8];  /* gam_NbCr, gam_NbNb, del_NbCr, del_NbNb, lav_NbCr, lav_NbNb,
+	                             phi_del_Nb, phi_lav_Nb */
+	const double* phi_del;
+	const double* phi_lav;
+} pf_mobility_fields;
+int pf_mobility_divergence_f64(int nx, int ny, double dx, double dy, const pf_mobility_fields* f,
+                               double* div_Cr, double* div_Nb, void* stream);
+
 /* Heuristic rows-per-block the step kernel uses for an nx x ny grid. */
 int pf_default_rows(int nx, int ny);
 /* Tile of the two-step (three-step) kernel: rows per chunk (0 = heuristic)
@@ -278,8 +305,18 @@ int pf_batch_read(pf_batch* batch, int job_id, void* conc_Cr, void* conc_Nb, voi
 int pf_batch_summarize(pf_batch* batch, int job_id, double* out5);
 /* Fails if any nucleation event needed an embed wider than one cell. */
 int pf_batch_status(pf_batch* batch, uint64_t* wide_events);
-/* Strip tasks per step of the active jobs (for throughput accounting). */
+/* Strip tasks per step of the active, unheld jobs (for throughput accounting). */
 int pf_batch_waves(pf_batch* batch);
+/* Hold (hold = 1) or release (0) a job: a held job keeps its fields and step
+ * count but takes no waves and does not advance until released, so the
+ * launches run only the other jobs (a scheduler's lever: e.g. run the jobs
+ * with the least remaining work first).  Takes effect from the next
+ * pf_batch_run; a job's results do not depend on when it was held. */
+int pf_batch_hold(pf_batch* batch, int job_id, int hold);
+/* The HIP stream (hipStream_t) every launch, copy and reduction of the batch
+ * is enqueued on, so callers can record events on it or order their own work
+ * after the batch's. */
+void* pf_batch_stream(pf_batch* batch);
 
 /* Host-side evaluation of the same pointwise thermodynamics (no GPU needed).
  * Replace the host functions of thermo/parabola625.h the driver calls:

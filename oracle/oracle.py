@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liborc.so")
+# PF_ORACLE_LIB: another build of pf_oracle.c (the CPU baseline's -O3
+# -march=native build, oracle/cpu_bench.py); the checker is liborc.so
+LIB = os.environ.get("PF_ORACLE_LIB") or os.path.join(HERE, "liborc.so")
 REF_LIB = os.path.join(HERE, "_ref", "libparabola625_ref.so")
 
 _d = ctypes.c_double
@@ -87,6 +89,10 @@ def lib():
                                      ctypes.POINTER(ctypes.c_ubyte)])
         L.orc_summarize.restype = None
         L.orc_summarize.argtypes = [_i, _i] + [_dp] * 5
+        L.orc_mobility_divergence.restype = None
+        L.orc_mobility_divergence.argtypes = [_i, _i, _d, _d, ctypes.POINTER(_dp) ,
+                                              ctypes.POINTER(_dp), ctypes.POINTER(_dp),
+                                              _dp, _dp, _dp, _dp]
         L.orc_energy.restype = None
         L.orc_energy.argtypes = [_i, _i, _i, _d, _d, _dp, _dp, _d] + [_dp] * 7
         L.orc_num_threads.restype = _i
@@ -309,3 +315,25 @@ def energy(fields, bc, dx, dy, kappa, omega, Omega, maps=False):
     if maps:
         return tuple(out), chem, grad
     return tuple(out)
+
+
+def mobility_divergence(conc, mob_cr, mob_nb, phi_del, phi_lav, dx, dy):
+    """orc_mobility_divergence: (div_Cr, div_Nb) over every cell of the padded
+    arrays.  conc: 6 arrays (gam/del/lav x Cr/Nb), mob_cr / mob_nb: 8 each
+    (terms 1-6 then the two phase terms)."""
+    c = [np.ascontiguousarray(a, dtype=np.float64) for a in conc]
+    mc = [np.ascontiguousarray(a, dtype=np.float64) for a in mob_cr]
+    mn = [np.ascontiguousarray(a, dtype=np.float64) for a in mob_nb]
+    pd = np.ascontiguousarray(phi_del, dtype=np.float64)
+    pl = np.ascontiguousarray(phi_lav, dtype=np.float64)
+    ny, nx = pd.shape
+    for a in c + mc + mn:
+        assert a.shape == (ny, nx)
+    cp = (_dp * 6)(*[_ptr(a) for a in c])
+    mcp = (_dp * 8)(*[_ptr(a) for a in mc])
+    mnp = (_dp * 8)(*[_ptr(a) for a in mn])
+    oc = np.empty((ny, nx))
+    on = np.empty((ny, nx))
+    lib().orc_mobility_divergence(nx, ny, dx, dy, cp, mcp, mnp, _ptr(pd), _ptr(pl), _ptr(oc),
+                                  _ptr(on))
+    return oc, on

@@ -19,6 +19,8 @@
  *   nucleation + embedding     src/discretization.cu:1046-1177
  *   diffusion matrix / dt      src/alloy625.cpp:80-121, src/main.cpp:259-268
  *   field summary              src/alloy625.cpp:811-863
+ *   variable-mobility CH       src/discretization.cu:605-778 (dead code there:
+ *   divergence                 its launch is commented out; mobilities are inputs)
  *
  * Pinning (what checks each function; see tests/):
  *   p, pPrime, g_*, dg_*, fict_*, D_*  golden vectors from the compiled reference
@@ -40,6 +42,11 @@
  *   field summary                     uniform interiors (exact p(0), p(1/2), p(1)),
  *                                     NaN / 1e300 ghost rings, dyadic fields with
  *                                     hand-computed means
+ *   mobility divergence               constant mobility = M times the five-point
+ *                                     Laplacian; linear mobility on quadratic fields
+ *                                     = the continuous divergence (exact for the
+ *                                     face average); zero interior sum with no-flux
+ *                                     and periodic ghosts; x-only phase terms
  *
  * Array layout: the reference's, (nx = Nx+2) x (ny = Ny+2) row-major FP64 with a
  * one-cell ghost ring, idx = nx*y + x.  Interior cells are 1..nx-2 by 1..ny-2.
@@ -392,13 +399,18 @@ void orc_run(int nx, int ny, int bc, const orc_params *P, int nsteps,
              double *bCr, double *bNb, double *bpd, double *bpl)
 {
 	const size_t n = (size_t)nx * ny;
+	double *src[4] = {aCr, aNb, apd, apl}, *dst[4] = {bCr, bNb, bpd, bpl};
 	for (int s = 0; s < nsteps; s++) {
-		orc_step(nx, ny, bc, P, aCr, aNb, apd, apl, bCr, bNb, bpd, bpl);
-		memcpy(aCr, bCr, n * sizeof(double));
-		memcpy(aNb, bNb, n * sizeof(double));
-		memcpy(apd, bpd, n * sizeof(double));
-		memcpy(apl, bpl, n * sizeof(double));
+		orc_step(nx, ny, bc, P, src[0], src[1], src[2], src[3], dst[0], dst[1], dst[2], dst[3]);
+		for (int k = 0; k < 4; k++) {
+			double *t = src[k];
+			src[k] = dst[k];
+			dst[k] = t;
+		}
 	}
+	if (src[0] != aCr)
+		for (int k = 0; k < 4; k++)
+			memcpy(dst[k], src[k], n * sizeof(double));
 }
 
 /* ------------------------------------------------------------------------- */
@@ -647,6 +659,67 @@ void orc_energy(int nx, int ny, int bc, double dx, double dy, const double kappa
 	out3[0] = (double)(sc + sg);
 	out3[1] = (double)sc;
 	out3[2] = (double)sg;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Variable-mobility Cahn-Hilliard divergence (src/discretization.cu:605-778) */
+/* ------------------------------------------------------------------------- */
+
+/* discrete_laplacian (src/discretization.cu:605-616): the face-averaged
+ * 1-D second difference ((D_hi + D_mid)(c_hi - c_mid)
+ * - (D_mid + D_low)(c_mid - c_low)) / (2 h^2). */
+static double orc_face_lap(double Dl, double Dm, double Dh, double cl, double cm, double ch,
+                           double h)
+{
+	return ((Dh + Dm) * (ch - cm) - (Dm + Dl) * (cm - cl)) / (2.0 * h * h);
+}
+
+/* chemical_convolution_Cr_kernel / _Nb_kernel (src/discretization.cu:618-778),
+ * evaluated at every cell of the padded (nx x ny) arrays, ghost ring
+ * included, with the reference's index clamp at the array edges (the left /
+ * bottom neighbour of column / row 0 and the right / top neighbour of the
+ * last column / row is the cell itself).  Per component i (Cr, Nb) the sum
+ * runs in the reference's order: terms 1-6, each in x then in y, over
+ * (mob[i][2a+j], conc[2a+j]) for phase a = gamma, delta, Laves and component
+ * j = Cr, Nb; then terms 7-8, (mob[i][6], phi_del) and (mob[i][7], phi_lav),
+ * in x ONLY (the reference has no y part for them, :688-695 and :769-776).
+ *   conc[6]    = x_gam_Cr, x_gam_Nb, x_del_Cr, x_del_Nb, x_lav_Cr, x_lav_Nb
+ *   mob[0][8]  = gam_CrCr, gam_CrNb, del_CrCr, del_CrNb, lav_CrCr, lav_CrNb,
+ *                phi_del_Cr, phi_lav_Cr
+ *   mob[1][8]  = gam_NbCr, gam_NbNb, del_NbCr, del_NbNb, lav_NbCr, lav_NbNb,
+ *                phi_del_Nb, phi_lav_Nb
+ * out[0] / out[1] receive div(M grad mu) for Cr / Nb (what the reference
+ * stores in conc_Cr_new / conc_Nb_new). */
+void orc_mobility_divergence(int nx, int ny, double dx, double dy, const double *const conc[6],
+                             const double *const mobCr[8], const double *const mobNb[8],
+                             const double *pd, const double *pl, double *outCr, double *outNb)
+{
+	const double *const *mob[2] = {mobCr, mobNb};
+	double *out[2] = {outCr, outNb};
+	#pragma omp parallel for schedule(static)
+	for (int y = 0; y < ny; y++) {
+		for (int x = 0; x < nx; x++) {
+			const size_t mid = (size_t)nx * y + x;
+			const size_t lft = (x == 0) ? mid : mid - 1;
+			const size_t rgt = (x == nx - 1) ? mid : mid + 1;
+			const size_t bot = (y == 0) ? mid : mid - nx;
+			const size_t top = (y == ny - 1) ? mid : mid + nx;
+			for (int i = 0; i < 2; i++) {
+				double s = 0.0;
+				for (int t = 0; t < 6; t++) {
+					const double *M = mob[i][t], *c = conc[t];
+					s += orc_face_lap(M[lft], M[mid], M[rgt], c[lft], c[mid], c[rgt], dx);
+					s += orc_face_lap(M[bot], M[mid], M[top], c[bot], c[mid], c[top], dy);
+				}
+				const double *phi[2] = {pd, pl};
+				for (int t = 0; t < 2; t++) {
+					const double *M = mob[i][6 + t], *c = phi[t];
+					s += orc_face_lap(M[lft], M[mid], M[rgt], c[lft], c[mid], c[rgt], dx);
+				}
+				out[i][mid] = s;
+			}
+		}
+	}
 }
 
 int orc_num_threads(void)

@@ -45,7 +45,7 @@ struct BatchJob {
   int parity;            // buffer set holding the current fields
   int adv;               // steps this launch advances the job (1, 2 or 3)
   int max_adv;           // steps per launch of the batch (1 where the grid is too small)
-  int pad;
+  int held;              // 1: the job takes no waves and does not advance (pf_batch_hold)
   StepParams P;
   NucParams N;
 };
@@ -60,7 +60,7 @@ struct BatchHeader {
 
 // the last step of this launch is one of the job's nucleation steps
 __host__ __device__ __forceinline__ bool nuc_due(const BatchJob& J) {
-  return J.active && J.nuc_every > 0 &&
+  return J.active && !J.held && J.nuc_every > 0 &&
          (J.step + (uint64_t)J.adv - 1) % (uint64_t)J.nuc_every == 0;
 }
 
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(256) void batch_advance_kernel(BatchJob* __restrict
   int due = 0;
   for (int j = threadIdx.x; j < hdr->njobs; j += blockDim.x) {
     BatchJob& J = jobs[j];
-    if (J.active) {
+    if (J.active && !J.held) {
       J.step += J.adv;
       J.parity ^= 1;
       if (J.step >= J.target) J.active = 0;
@@ -610,9 +610,9 @@ int pf_batch_commit(pf_batch* b) {
     J.max_adv = (J.Nx >= 3 && J.Ny >= 3) ? b->launch_steps : 1;
     J.adv = J.active ? pf::next_adv(J) : 1;
     J.wave0 = (int)wave_job.size();
-    if (!J.active) {
+    if (!J.active || J.held) {
       J.wave0 = 0;
-      continue;  // finished jobs take no waves
+      continue;  // finished and held jobs take no waves
     }
     const long nw = (long)J.strips * J.chunks;
     if ((long)wave_job.size() + nw > b->max_waves) return bfail(PF_ERR_OVERFLOW, "wave table full");
@@ -765,11 +765,25 @@ int pf_batch_status(pf_batch* b, uint64_t* wide_events) {
   return PF_OK;
 }
 
+int pf_batch_hold(pf_batch* b, int job_id, int hold) {
+  if (!b) return bfail(PF_ERR_ARG, "null batch");
+  if (job_id < 0 || job_id >= (int)b->jobs.size() || !b->slot_live[job_id])
+    return bfail(PF_ERR_ARG, "no such job");
+  const int h = hold ? 1 : 0;
+  if (b->jobs[job_id].held == h) return PF_OK;
+  if (int r = sync_jobs_from_device(b)) return r;
+  b->jobs[job_id].held = h;
+  b->dirty = true;
+  return PF_OK;
+}
+
+void* pf_batch_stream(pf_batch* b) { return b ? (void*)b->stream : nullptr; }
+
 int pf_batch_waves(pf_batch* b) {
   if (!b) return 0;
   int total = 0;
   for (size_t j = 0; j < b->jobs.size(); j++)
-    if (b->slot_live[j] && b->jobs[j].active)
+    if (b->slot_live[j] && b->jobs[j].active && !b->jobs[j].held)
       total += ((b->jobs[j].Nx + pf::STRIP3 - 1) / pf::STRIP3) *
                ((b->jobs[j].Ny + b->rows - 1) / b->rows);
   return total;

@@ -41,6 +41,11 @@ class PfStepItem(ctypes.Structure):
                 ("new_fields", ctypes.c_void_p * 4)]
 
 
+class PfMobilityFields(ctypes.Structure):
+    _fields_ = [("conc", _vp * 6), ("mob_Cr", _vp * 8), ("mob_Nb", _vp * 8),
+                ("phi_del", _vp), ("phi_lav", _vp)]
+
+
 class PfError(RuntimeError):
     pass
 
@@ -79,6 +84,8 @@ PROTOTYPES = {
     "pf_step_loop_f64": (_i, [_vp, _i, ctypes.POINTER(PfStepItem), _vp]),
     "pf_step2_loop_f64": (_i, [_vp, _i, ctypes.POINTER(PfStepItem), _vp]),
     "pf_step3_loop_f64": (_i, [_vp, _i, ctypes.POINTER(PfStepItem), _vp]),
+    "pf_mobility_divergence_f64": (_i, [_i, _i, _d, _d, ctypes.POINTER(PfMobilityFields), _vp,
+                                        _vp, _vp]),
     "pf_host_p": (_d, [_d]),
     "pf_host_pPrime": (_d, [_d]),
     "pf_host_g": (_d, [_i, _d, _d]),

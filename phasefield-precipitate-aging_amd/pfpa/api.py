@@ -367,3 +367,42 @@ def nucleation_status(dev):
     rc = lib().pf_nucleation_status(dev.ctx, ctypes.byref(wide), ctypes.byref(over))
     check(rc, "pf_nucleation_status")
     return wide.value, bool(over.value)
+
+
+def chemical_convolution(conc, mob_Cr, mob_Nb, phi_del, phi_lav, dx, dy, div_Cr=None,
+                         div_Nb=None, stream=None):
+    """The variable-mobility Cahn-Hilliard divergence of
+    chemical_convolution_Cr_kernel / chemical_convolution_Nb_kernel
+    (src/discretization.cu:605-778) on device tensors: conc = the six
+    fictitious compositions (gam, del, lav x Cr, Nb), mob_Cr / mob_Nb = the
+    eight mobility arrays of each component (terms 1-6, then the delta and
+    Laves phase terms, which are differenced in x only), every array a CUDA
+    FP64 tensor of the padded (ny, nx) shape with its ghost ring filled.
+    Returns (div_Cr, div_Nb) over every cell.  The reference never launches
+    these kernels (src/discretization.cu:809-835) and leaves the mobility
+    functions d_M_* undefined, so the mobilities are the caller's."""
+    arrs = list(conc) + list(mob_Cr) + list(mob_Nb) + [phi_del, phi_lav]
+    if len(conc) != 6 or len(mob_Cr) != 8 or len(mob_Nb) != 8:
+        raise ValueError("need 6 compositions and 8 + 8 mobility arrays")
+    shape = phi_del.shape
+    for a in arrs:
+        if a.dtype != torch.float64 or not a.is_cuda or a.shape != shape or \
+                not a.is_contiguous():
+            raise ValueError("every array must be a contiguous CUDA float64 tensor of one shape")
+    ny, nx = shape
+    if div_Cr is None:
+        div_Cr = torch.empty_like(phi_del)
+    if div_Nb is None:
+        div_Nb = torch.empty_like(phi_del)
+    f = _lib.PfMobilityFields()
+    for k in range(6):
+        f.conc[k] = conc[k].data_ptr()
+    for k in range(8):
+        f.mob_Cr[k] = mob_Cr[k].data_ptr()
+        f.mob_Nb[k] = mob_Nb[k].data_ptr()
+    f.phi_del = phi_del.data_ptr()
+    f.phi_lav = phi_lav.data_ptr()
+    st = ctypes.c_void_p(stream.cuda_stream) if stream is not None else _stream()
+    check(lib().pf_mobility_divergence_f64(nx, ny, dx, dy, ctypes.byref(f), _ptr(div_Cr),
+                                           _ptr(div_Nb), st), "pf_mobility_divergence_f64")
+    return div_Cr, div_Nb

@@ -43,6 +43,8 @@ PROTOS = {
     "pf_batch_summarize": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_double)]),
     "pf_batch_status": (_i, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
     "pf_batch_waves": (_i, [_vp]),
+    "pf_batch_hold": (_i, [_vp, _i, _i]),
+    "pf_batch_stream": (_vp, [_vp]),
 }
 
 
@@ -183,10 +185,26 @@ class Batch:
         check(blib().pf_batch_summarize(self.h, jid, o), "pf_batch_summarize")
         return list(o)
 
-    def status(self):
+    def status(self, raise_on_wide=True):
+        """Nucleation events that needed an embed wider than one cell (batch
+        mode counts them instead of applying them).  Raises PfError when there
+        were any, unless raise_on_wide is False."""
         w = ctypes.c_uint64()
-        check(blib().pf_batch_status(self.h, ctypes.byref(w)), "pf_batch_status")
+        rc = blib().pf_batch_status(self.h, ctypes.byref(w))
+        if rc == 1 and w.value and not raise_on_wide:   # PF_ERR_ARG: wide events counted
+            return w.value
+        check(rc, "pf_batch_status")
         return w.value
+
+    def hold(self, jid, on=True):
+        """Hold (or release) a job: it keeps its state but takes no part in
+        the launches until released."""
+        check(blib().pf_batch_hold(self.h, jid, 1 if on else 0), "pf_batch_hold")
+
+    def stream(self):
+        """The batch's hipStream_t handle (an int), e.g. for
+        torch.cuda.ExternalStream."""
+        return blib().pf_batch_stream(self.h)
 
     def waves(self):
         return blib().pf_batch_waves(self.h)

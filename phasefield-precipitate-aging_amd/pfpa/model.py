@@ -212,6 +212,26 @@ def init_matrix_with_particles(Nx, Ny, dx, bc, seed=1, xCr0=0.3041, xNb0=0.0235,
     pl = np.zeros((ny, nx))
     xCr[1:-1, 1:-1] = xCr0 + noise * rng.standard_normal((Ny, Nx))
     xNb[1:-1, 1:-1] = xNb0 + noise * rng.standard_normal((Ny, Nx))
+    _stamp_particles(rng, [xCr, xNb, pd, pl], dx, bc, n_del, n_lav, radius)
+    for a in (xCr, xNb, pd, pl):
+        fill_ghosts(a, bc)
+    return [xCr, xNb, pd, pl]
+
+
+def add_particles(fields, dx, bc, seed=1, n_del=2, n_lav=2, radius=5e-9):
+    """Stamp seeded delta and Laves discs (phi = 1 and the equilibrium
+    compositions inside) onto padded fields (x_Cr, x_Nb, phi_del, phi_lav)
+    in place, and refresh their ghost rings."""
+    rng = np.random.default_rng(seed)
+    _stamp_particles(rng, fields, dx, bc, n_del, n_lav, radius)
+    for a in fields:
+        fill_ghosts(a, bc)
+    return fields
+
+
+def _stamp_particles(rng, fields, dx, bc, n_del, n_lav, radius):
+    xCr, xNb, pd, pl = fields
+    Ny, Nx = xCr.shape[0] - 2, xCr.shape[1] - 2
     R = radius / dx
     dels, cs = _place_discs(rng, Nx, Ny, R, n_del, 2 * R + 8)
     lavs, cs = _place_discs(rng, Nx, Ny, R, n_lav, 2 * R + 8, cs)
@@ -222,6 +242,7 @@ def init_matrix_with_particles(Nx, Ny, dx, bc, seed=1, xCr0=0.3041, xNb0=0.0235,
         return np.unique(i % n) if bc == 1 else i[(i >= 0) & (i < n)]
 
     for centres, phi, ph in ((dels, pd, 1), (lavs, pl, 2)):
+        xe = (_xe(ph, 0), _xe(ph, 1))
         for (cx, cy) in centres:
             xs, ys = box(cx, Nx), box(cy, Ny)
             # per cell the same arithmetic as over the whole grid, on the box only
@@ -234,11 +255,8 @@ def init_matrix_with_particles(Nx, Ny, dx, bc, seed=1, xCr0=0.3041, xNb0=0.0235,
             rows, cols = np.broadcast_arrays(ys[:, None] + 1, xs[None, :] + 1)
             rows, cols = rows[inside], cols[inside]
             phi[rows, cols] = 1.0
-            xCr[rows, cols] = _xe(ph, 0)
-            xNb[rows, cols] = _xe(ph, 1)
-    for a in (xCr, xNb, pd, pl):
-        fill_ghosts(a, bc)
-    return [xCr, xNb, pd, pl]
+            xCr[rows, cols] = xe[0]
+            xNb[rows, cols] = xe[1]
 
 
 def init_enriched_band(Nx, Ny, bc, seed=42, dtype=np.float64):

@@ -81,20 +81,39 @@ def request_offset(r):
 
 def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_steps=3,
           log=None, storage=PF_STORAGE_F32, time_limit=None, read_fields=False,
-          offset=True):
+          offset=True, order="fifo", fill_cells=16 << 20, max_hold_s=2.0, max_bypass_s=1.0):
     """Run the request stream; returns (finished requests, stats).  Each device
     round is one replay of a graph of `launches_per_graph` step launches, each
     advancing every running job by up to `launch_steps` steps (1, 2 or 3).
     read_fields: also keep each request's final fields (r.fields, absolute
     compositions).  offset (FP32 storage): store each request's compositions
     relative to its matrix composition (request_offset), which keeps more bits
-    of each step's increment."""
+    of each step's increment.
+
+    order "fifo": waiting requests are admitted in arrival order.
+    order "srw" (shortest remaining work first): waiting requests are admitted
+    in order of work (cells x steps), and every round only the running jobs
+    with the least remaining work run, as many as fill `fill_cells` cells per
+    launch; the others are held for the round (pf_batch_hold), except that a
+    job held for `max_hold_s` in a row runs from then on until it is done.
+    Results do not depend on the order: a held job resumes where it stopped.
+    In both orders a request that does not fit in the pool lets the ones
+    behind it pass, but once it has waited `max_bypass_s` since it first did
+    not fit, nothing behind it is admitted until it fits (a large request is
+    not starved by a stream of small ones)."""
+    if order not in ("fifo", "srw"):
+        raise ValueError("order must be 'fifo' or 'srw'")
+    import torch
     batch = Batch(storage, pool_bytes=pool_bytes, max_jobs=max(16, len(requests)))
     batch.set_tile(rows, depth)
     batch.set_launch_steps(launch_steps)
+    ext = torch.cuda.ExternalStream(batch.stream())
     pending = list(requests)       # sorted by arrival
     waiting = []
     running = {}                   # job id -> request
+    progress = {}                  # job id -> steps done at the last poll
+    held_since = {}                # job id -> time it was first held (in a row)
+    promoted = set()               # jobs held for max_hold_s: never held again
     finished = []
     rejected = []                  # larger than the whole pool
     # rid -> future of (desc, fields), built on a worker thread while the loop
@@ -102,6 +121,9 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
     prepared = {}
     workers = ThreadPoolExecutor(max_workers=1)
     cell_updates = 0
+    busy = 0.0                     # device time of the rounds (HIP events on the batch stream)
+    rounds = held_rounds = 0
+    blocked_since = {}             # rid -> time it first did not fit
     t0 = time.perf_counter()
 
     def build(r):
@@ -122,11 +144,16 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
             break
         while pending and pending[0].arrival <= now:
             waiting.append(pending.pop(0))
-        # admit in arrival order; a request that does not fit lets smaller ones
-        # pass, and one whose fields are still being built waits for a later
-        # round while the device has work
+        if order == "srw":
+            waiting.sort(key=lambda q: (q.n * q.n * q.steps, q.arrival))
+        # admit; a request whose fields are still being built waits for a
+        # later round while the device has work
         still = []
+        barrier = False            # a request passed over too long blocks the rest
         for r in waiting:
+            if barrier:
+                still.append(r)
+                continue
             fut = prepare(r)
             if running and not fut.done():
                 still.append(r)
@@ -140,22 +167,57 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
                     rejected.append(r)
                 else:
                     still.append(r)
+                    first = blocked_since.setdefault(r.rid, now)
+                    if now - first > max_bypass_s:
+                        barrier = True
                 continue
             del prepared[r.rid]
+            blocked_since.pop(r.rid, None)
             r.job = jid
             r.admitted = time.perf_counter() - t0
             running[jid] = r
+            progress[jid] = 0
         waiting = still
         if running:
+            if order == "srw":
+                now = time.perf_counter() - t0
+                ranked = sorted(running, key=lambda j: (running[j].n ** 2 *
+                                                        (running[j].steps - progress[j]),
+                                                        running[j].arrival))
+                cells = 0
+                for j in ranked:
+                    # a job held too long runs from then on until it is done
+                    if j in held_since and now - held_since[j] > max_hold_s:
+                        promoted.add(j)
+                    run = cells < fill_cells or j in promoted
+                    if run:
+                        cells += running[j].n ** 2
+                        if j in held_since:
+                            batch.hold(j, False)
+                            del held_since[j]
+                    else:
+                        if j not in held_since:
+                            batch.hold(j, True)
+                            held_since[j] = now
+                        held_rounds += 1
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(ext)
             batch.run(launches_per_graph, launches_per_graph)   # asynchronous
+            e1.record(ext)
+            rounds += 1
             # start building the next admissions' fields
             now = time.perf_counter() - t0
             for r in waiting[:4] + [q for q in pending[:8] if q.arrival <= now + 0.5]:
                 prepare(r)
-            states = batch.poll_all()
+            states = batch.poll_all()     # waits for the round
+            busy += e0.elapsed_time(e1) / 1e3
             for jid, steps, active in states:
                 r = running.get(jid)
-                if r is None or active:
+                if r is None:
+                    continue
+                progress[jid] = steps
+                if active:
                     continue
                 r.summary = batch.summarize(jid)
                 if read_fields:
@@ -164,6 +226,9 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
                 cell_updates += r.n * r.n * r.steps
                 batch.remove(jid)
                 del running[jid]
+                progress.pop(jid, None)
+                held_since.pop(jid, None)
+                promoted.discard(jid)
                 finished.append(r)
                 if log:
                     log(r)
@@ -173,19 +238,36 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
             time.sleep(max(0.0, min(0.01, pending[0].arrival - (time.perf_counter() - t0))))
     wall = time.perf_counter() - t0
     workers.shutdown(wait=True, cancel_futures=True)
-    batch.status()
+    # nucleation events wider than one cell are counted, not applied, in batch
+    # mode (they need a domain of a few cells); report them with the results
+    wide = batch.status(raise_on_wide=False)
     batch.close()
     lat = np.array([r.done - r.arrival for r in finished]) if finished else np.zeros(1)
+    by_size = {}
+    for name, sel in (("side<=512", lambda r: r.n <= 512), ("side>512", lambda r: r.n > 512)):
+        ls = np.array([r.done - r.arrival for r in finished if sel(r)])
+        by_size[name] = ({"count": int(ls.size), "p50_s": float(np.percentile(ls, 50)),
+                          "p99_s": float(np.percentile(ls, 99))} if ls.size else {"count": 0})
+    window = max((r.arrival for r in requests), default=0.0)
     stats = {
         "requests": len(requests),
         "completed": len(finished),
         "rejected": len(rejected),
+        "order": order,
         "wall_s": wall,
         "cell_updates": cell_updates,
         "throughput_cell_updates_per_s": cell_updates / wall if wall > 0 else 0.0,
+        "offered_cell_updates_per_s": (sum(r.n * r.n * r.steps for r in requests) / window
+                                       if window > 0 else None),
+        "device_busy_s": busy,
+        "device_busy_share": busy / wall if wall > 0 else 0.0,
+        "rounds": rounds,
+        "held_job_rounds": held_rounds,
         "latency_p50_s": float(np.percentile(lat, 50)),
         "latency_p99_s": float(np.percentile(lat, 99)),
         "latency_max_s": float(lat.max()),
+        "latency_by_size": by_size,
+        "wide_nucleation_events": int(wide),
         "launches_per_graph": launches_per_graph,
         "launch_steps": launch_steps,
         "pool_bytes": pool_bytes,

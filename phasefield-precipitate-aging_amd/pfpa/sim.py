@@ -158,8 +158,11 @@ class Simulation:
             raise ValueError("tune_tile3 needs three steps per launch")
         fn = self._stepk[3]
         st = ctypes.c_void_p(self.stream.cuda_stream)
-        a = [x.clone() for x in self.a]
-        b = [torch.empty_like(x) for x in a]
+        # the copies are made on the simulation's stream, after the steps
+        # queued there, and the kernels that read them run there too
+        with torch.cuda.stream(self.stream):
+            a = [x.clone() for x in self.a]
+            b = [torch.empty_like(x) for x in a]
 
         def run(n):
             for i in range(n):
@@ -218,7 +221,11 @@ class Simulation:
         return out
 
     def summarize(self):
-        out = torch.zeros(5, dtype=torch.float64, device="cuda")
+        # allocated and cleared on the simulation's stream, in order with the
+        # reduction that writes it (a stream of its own need not wait for the
+        # current one)
+        with torch.cuda.stream(self.stream):
+            out = torch.zeros(5, dtype=torch.float64, device="cuda")
         a = self.a
         check(self._summ(self.ctx, self.nx, self.ny, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]),
                          _ptr(a[3]), _ptr(out), ctypes.c_void_p(self.stream.cuda_stream)),
@@ -235,11 +242,12 @@ class Simulation:
         s = self.s
         kappa = (ctypes.c_double * 2)(*s.kappa)
         omega = (ctypes.c_double * 2)(*s.omega)
-        out = torch.zeros(3, dtype=torch.float64, device="cuda")
         chem = grad = None
-        if maps:
-            chem = torch.zeros(self.nx * self.ny, dtype=torch.float64, device="cuda")
-            grad = torch.zeros_like(chem)
+        with torch.cuda.stream(self.stream):
+            out = torch.zeros(3, dtype=torch.float64, device="cuda")
+            if maps:
+                chem = torch.zeros(self.nx * self.ny, dtype=torch.float64, device="cuda")
+                grad = torch.zeros_like(chem)
         a = self.a
         check(fn(self.ctx, self.nx, self.ny, s.bc, s.dx, s.dx, kappa, omega, s.alpha,
                  _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(a[3]),

@@ -244,12 +244,15 @@ def test_concurrent_simulations_on_two_threads(gpu):
     inputs = [M.init_matrix_with_particles(s.N[0], s.N[1], s.dx, s.bc, seed=seed,
                                            radius=2e-9) for s, seed in cases]
     alone = []
+    alone_summ = []
     for (s, _), f in zip(cases, inputs):
         sim = Simulation(s, f, nucleation=True, seed=5)
         sim.step(40)
         alone.append(sim.fields())
+        alone_summ.append(sim.summarize())
         sim.close()
     out = [None, None]
+    summ = [None, None]
     errors = []
 
     def worker(i):
@@ -259,6 +262,9 @@ def test_concurrent_simulations_on_two_threads(gpu):
             sim = Simulation(s, inputs[i], nucleation=True, seed=5, stream=stream)
             for _ in range(10):
                 sim.step(4)
+            # queued behind the steps on the simulation's own stream: the
+            # result buffer is made there too, not on the current stream
+            summ[i] = sim.summarize()
             out[i] = sim.fields()
             sim.close()
         except Exception as e:  # reported below
@@ -273,3 +279,4 @@ def test_concurrent_simulations_on_two_threads(gpu):
     for i in range(2):
         for k in range(4):
             assert np.array_equal(out[i][k], alone[i][k]), (i, k)
+        assert summ[i] == alone_summ[i], i

@@ -56,3 +56,57 @@ def test_request_larger_than_the_pool_is_rejected():
     done, stats = serve(reqs + [big], pool, launches_per_graph=3, rows=16)
     assert stats["rejected"] == 1 and stats["completed"] == len(reqs)
     assert big not in done
+
+
+def test_hold_keeps_state_and_resumes_same_bits():
+    """A held job takes no part in the launches (its step count stays) and,
+    released, ends with the same bits as a job never held."""
+    from pfpa import model as M
+    from pfpa.batch import Batch, make_desc
+    s = M.make_setup(90, 70, bc=1)
+    f = M.init_matrix_with_particles(90, 70, s.dx, 1, seed=4, radius=2e-9)
+    so = M.make_setup(64, 64, bc=0)
+    fo = M.init_matrix_with_particles(64, 64, so.dx, 0, seed=5, radius=2e-9)
+
+    def run(hold):
+        b = Batch(PF_STORAGE_F32, pool_bytes=64 << 20)
+        b.set_launch_steps(3)
+        ja = b.add(make_desc(s, 40, nucleation=True, seed=9, instance=1), f)
+        jo = b.add(make_desc(so, 200, instance=2), fo)
+        b.run(2, 0)
+        if hold:
+            at = b.poll(ja)[0]
+            b.hold(ja, True)
+            b.run(5, 5)                      # graph replays while held
+            assert b.poll(ja)[0] == at       # no progress while held
+            assert b.poll(jo)[0] > at        # the other job ran
+            b.hold(ja, False)
+        b.run(30, 3)
+        assert b.poll(ja) == (40, False)
+        out = b.read(ja)
+        b.close()
+        return out
+
+    a, h = run(False), run(True)
+    for k in range(4):
+        assert np.array_equal(a[k], h[k]), k
+
+
+def test_serving_shortest_remaining_work_same_results():
+    """Shortest-remaining-work order holds the longer jobs (some rounds run
+    only the shorter ones) and every result is the same bits as in FIFO."""
+    reqs = make_requests(rate=300.0, duration=0.03, seed=9, n_lo=64, n_hi=200, steps_lo=20,
+                         steps_hi=120)
+    assert len(reqs) >= 6
+    pool = 1 << 30
+    fifo, st_f = serve(reqs, pool, launches_per_graph=3, rows=16, read_fields=True)
+    srw, st_s = serve(reqs, pool, launches_per_graph=3, rows=16, read_fields=True,
+                      order="srw", fill_cells=20000, max_hold_s=0.05)
+    assert st_f["completed"] == st_s["completed"] == len(reqs)
+    assert st_s["held_job_rounds"] > 0 and st_f["held_job_rounds"] == 0
+    assert 0.0 < st_s["device_busy_share"] <= 1.0
+    by_f = {r.rid: r for r in fifo}
+    for r in srw:
+        for k in range(4):
+            assert np.array_equal(r.fields[k], by_f[r.rid].fields[k]), (r.rid, k)
+        assert r.summary == by_f[r.rid].summary

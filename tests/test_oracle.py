@@ -639,3 +639,109 @@ def test_energy_gradient_closed_forms(oracle):
     _, _, grad = oracle.energy(f, 1, dx, dy, E_KAPPA, E_OMEGA, E_OMEGA3)
     want = E_KAPPA[1] * dx * dy * Nx * (A * np.sin(th) / dy) ** 2 * Ny / 2
     assert grad == pytest.approx(want, rel=1e-12)
+
+
+# ------------------------------------------------ variable-mobility divergence
+
+
+def _mob_case(Nx, Ny, seed):
+    rng = np.random.default_rng(seed)
+    shape = (Ny + 2, Nx + 2)
+    conc = [rng.uniform(0.0, 0.5, shape) for _ in range(6)]
+    mob_cr = [rng.uniform(0.5, 2.0, shape) for _ in range(8)]
+    mob_nb = [rng.uniform(0.5, 2.0, shape) for _ in range(8)]
+    phis = [rng.uniform(0.0, 1.0, shape) for _ in range(2)]
+    return conc, mob_cr, mob_nb, phis
+
+
+def _fill(a, bc):
+    """Ghost ring: no-flux copy of the adjacent interior cell (corners from the
+    diagonal one) or periodic wrap."""
+    a = a.copy()
+    if bc == 0:
+        a[0, :], a[-1, :] = a[1, :], a[-2, :]
+        a[:, 0], a[:, -1] = a[:, 1], a[:, -2]
+    else:
+        a[0, :], a[-1, :] = a[-2, :], a[1, :]
+        a[:, 0], a[:, -1] = a[:, -2], a[:, 1]
+    return a
+
+
+def test_mobility_divergence_constant_mobility_is_laplacian(oracle):
+    """Constant mobility M on one composition term (all others zero): the
+    face average (2M)(c_hi - c) - (2M)(c - c_lo) over 2 h^2 is M times the
+    five-point Laplacian, at every interior cell (dx != dy)."""
+    Nx, Ny, dx, dy = 13, 9, 0.5, 0.25
+    conc, _, _, phis = _mob_case(Nx, Ny, 1)
+    z = np.zeros_like(conc[0])
+    for t in range(6):
+        for comp in range(2):
+            mc = [z] * 8
+            mn = [z] * 8
+            M = 1.75
+            (mc if comp == 0 else mn)[t] = np.full_like(z, M)
+            out = oracle.mobility_divergence(conc, mc, mn, phis[0], phis[1], dx, dy)
+            c = conc[t]
+            lap = ((c[1:-1, 2:] - 2 * c[1:-1, 1:-1] + c[1:-1, :-2]) / dx ** 2 +
+                   (c[2:, 1:-1] - 2 * c[1:-1, 1:-1] + c[:-2, 1:-1]) / dy ** 2)
+            got = out[comp][1:-1, 1:-1]
+            assert np.allclose(got, M * lap, rtol=1e-13, atol=1e-13 * np.abs(M * lap).max())
+            assert not np.any(out[1 - comp])           # the other component sees nothing
+
+
+def test_mobility_divergence_linear_mobility_quadratic_field(oracle):
+    """M = a + b x (linear) and c = x^2: the face-averaged difference equals
+    the continuous d/dx (M dc/dx) = 2a + 4 b x exactly (up to rounding); the
+    same in y.  A transposed or mis-paired face average fails this."""
+    Nx, Ny, h = 20, 16, 0.125
+    X = (np.arange(Nx + 2) * h)[None, :] * np.ones((Ny + 2, 1))
+    Y = (np.arange(Ny + 2) * h)[:, None] * np.ones((1, Nx + 2))
+    a, b = 1.5, 0.75
+    z = np.zeros_like(X)
+    for axis, coord in ((0, X), (1, Y)):
+        conc = [coord ** 2] + [z] * 5
+        mc = [a + b * coord] + [z] * 7
+        out = oracle.mobility_divergence(conc, mc, [z] * 8, z, z, h, h)[0]
+        want = 2 * a + 4 * b * coord
+        assert np.allclose(out[1:-1, 1:-1], want[1:-1, 1:-1], rtol=1e-12, atol=0)
+
+
+def test_mobility_divergence_phase_terms_are_x_only(oracle):
+    """Terms 7 and 8 (the phase-field terms) difference in x only, as the
+    reference's kernels do: a phase field varying in y alone contributes
+    nothing; one varying in x alone gives the 1-D face average."""
+    Nx, Ny, dx, dy = 11, 12, 0.5, 0.5
+    z = np.zeros((Ny + 2, Nx + 2))
+    Yv = (np.arange(Ny + 2, dtype=float)[:, None] ** 2) * np.ones((1, Nx + 2))
+    Xv = (np.arange(Nx + 2, dtype=float)[None, :] ** 2) * np.ones((Ny + 2, 1))
+    M = np.full_like(z, 2.0)
+    for slot, which in ((6, 0), (7, 1)):
+        mc = [z] * 8
+        mc[slot] = M
+        phis = [z, z]
+        phis[which] = Yv
+        out = oracle.mobility_divergence([z] * 6, mc, [z] * 8, phis[0], phis[1], dx, dy)[0]
+        assert not np.any(out[1:-1, 1:-1])
+        phis[which] = Xv
+        out = oracle.mobility_divergence([z] * 6, mc, [z] * 8, phis[0], phis[1], dx, dy)[0]
+        # (4 (c_hi - c) - 4 (c - c_lo)) / (2 dx^2) with c = x^2: 2 * 2 / dx^2 * 2 / 2
+        assert np.allclose(out[1:-1, 1:-1], 2.0 * 2.0 / dx ** 2)
+
+
+@pytest.mark.parametrize("bc", [0, 1])
+def test_mobility_divergence_conserves_mass(oracle, bc):
+    """With the ghost ring filled by the no-flux copy or the periodic wrap
+    (fields and mobilities alike), the fluxes through the domain edge vanish
+    or cancel: the interior sum of the divergence is zero to rounding."""
+    Nx, Ny = 17, 10
+    conc, mc, mn, phis = _mob_case(Nx, Ny, 5)
+    conc = [_fill(a, bc) for a in conc]
+    mc = [_fill(a, bc) for a in mc]
+    mn = [_fill(a, bc) for a in mn]
+    phis = [_fill(a, bc) for a in phis]
+    dx, dy = 0.5, 0.4
+    out = oracle.mobility_divergence(conc, mc, mn, phis[0], phis[1], dx, dy)
+    for o in out:
+        interior = o[1:-1, 1:-1]
+        assert abs(interior.sum()) <= 1e-12 * np.abs(interior).sum()
+        assert np.abs(interior).max() > 1.0        # not trivially zero
