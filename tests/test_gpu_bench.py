@@ -34,27 +34,38 @@ def test_bench_json_line():
     assert rf["bound"] == "hbm" and rf["unit"] == "GB/s" and rf["peak"] == 8000.0
     assert rf["frac"] == pytest.approx(rf["achieved"] / rf["peak"], rel=1e-12)
     assert d["cpu_baseline"] is None
+    # the timed loop and the kernel leg ran in the same clock state
+    t = d["timing"]
+    assert t["three_step_kernel_us_per_step"] <= t["timed_loop_us_per_step"]
 
 
 
 def test_bench_precision_form_line():
-    """bench.py --form precision: one JSON line with the error rows at each mark
-    for FP32 storage with and without the composition offset, and the rates."""
+    """bench.py --form precision: one JSON line with, for the no-precipitate
+    and the seeded-precipitate workloads, the error rows at each mark (FP32
+    storage with and without the composition offset against FP64) and the
+    rates."""
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--form", "precision",
-                        "--size", "256", "--precision-marks", "10", "30"],
+                        "--precision-size", "256", "--precision-marks", "10", "30",
+                        "--precision-particles", "2"],
                        capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stderr[-2000:]
     lines = [l for l in r.stdout.splitlines() if l.strip()]
     assert len(lines) == 1
-    d = json.loads(lines[0])
-    assert d["form"] == "precision" and d["grid"] == [256, 256]
-    assert [row["step"] for row in d["rows"]] == [10, 30]
-    for row in d["rows"]:
-        for key in ("max_abs_dx_Cr", "max_abs_dx_Nb", "max_abs_dx_Cr_offset",
-                    "max_abs_dx_Nb_offset", "mass_drift_Cr_32", "mass_drift_Cr_32_offset"):
+    d = json.loads(lines[0])["precision"]
+    assert d["grid"] == [256, 256]
+    plain, parts = d["no_precipitates"], d["with_precipitates"]
+    assert [row["step"] for row in plain["rows"]] == [10, 30]
+    for row in plain["rows"]:
+        for key in ("max_abs_dx_Cr_fp32", "max_abs_dx_Nb_fp32", "max_abs_dx_Cr_fp32_offset",
+                    "max_abs_dx_Nb_fp32_offset", "mass_drift_Cr_fp32",
+                    "mass_drift_Cr_fp32_offset", "mass_drift_Cr_fp64"):
             assert key in row, key
-        assert 0 < row["max_abs_dx_Cr_offset"] < row["max_abs_dx_Cr"] < 1e-5
-    rates = d["rates"]
-    for key in ("fp64", "fp32_storage", "fp32_storage_offset"):
-        assert rates[key] > 0, key
-    assert rates["steps"] == 30 and len(rates["offset"]) == 2
+        assert 0 < row["max_abs_dx_Cr_fp32_offset"] < row["max_abs_dx_Cr_fp32"] < 1e-5
+    for key in ("fp64", "fp32", "fp32_offset"):
+        assert plain["rates_cell_updates_per_s"][key] > 0, key
+    assert plain["steps"] == 30 and len(plain["offsets"]["fp32_offset"]) == 2
+    # with precipitates the fractions are not zero and the FP32 error is small
+    for row in parts["rows"]:
+        assert row["delta_frac_fp64"] > 1e-3 and row["laves_frac_fp64"] > 1e-3
+        assert row["phase_frac_abs_err_fp32_offset"] < 1e-6
