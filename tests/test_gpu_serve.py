@@ -95,9 +95,9 @@ def test_hold_keeps_state_and_resumes_same_bits():
 def test_serving_shortest_remaining_work_same_results():
     """Shortest-remaining-work order holds the longer jobs (some rounds run
     only the shorter ones) and every result is the same bits as in FIFO."""
-    reqs = make_requests(rate=300.0, duration=0.03, seed=9, n_lo=64, n_hi=200, steps_lo=20,
-                         steps_hi=120)
-    assert len(reqs) >= 6
+    reqs = make_requests(rate=300.0, duration=1.0, seed=9, n_lo=64, n_hi=200, steps_lo=20,
+                         steps_hi=120, max_requests=10)
+    assert len(reqs) == 10
     pool = 1 << 30
     fifo, st_f = serve(reqs, pool, launches_per_graph=3, rows=16, read_fields=True)
     srw, st_s = serve(reqs, pool, launches_per_graph=3, rows=16, read_fields=True,
