@@ -98,6 +98,8 @@ def test_serving_shortest_remaining_work_same_results():
     reqs = make_requests(rate=300.0, duration=1.0, seed=9, n_lo=64, n_hi=200, steps_lo=20,
                          steps_hi=120, max_requests=10)
     assert len(reqs) == 10
+    for r in reqs:          # all waiting at once, so several run together
+        r.arrival = 0.0
     pool = 1 << 30
     fifo, st_f = serve(reqs, pool, launches_per_graph=3, rows=16, read_fields=True)
     srw, st_s = serve(reqs, pool, launches_per_graph=3, rows=16, read_fields=True,
