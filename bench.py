@@ -489,12 +489,15 @@ def serving_section(args, capacity=None):
         finished, stats = serve(reqs, pool_bytes=args.serve_pool_gb << 30,
                                 launches_per_graph=args.serve_lpg, rows=args.serve_rows,
                                 depth=3, launch_steps=3, log=progress, order=order)
-        sample = sorted(finished, key=lambda r: r.n * r.n * r.steps)[:3]
-        err, _ = fp64_replay_error(finished, sample)
+        # FP64 replays of the requests with the least, the median and the most
+        # work (cells x steps)
+        by_work = sorted(finished, key=lambda r: r.n * r.n * r.steps)
+        sample = [by_work[0], by_work[len(by_work) // 2], by_work[-1]] if by_work else []
+        err, rows = fp64_replay_error(finished, sample)
         stats["arrival_rate_per_s"] = rate
         stats["arrival_window_s"] = args.serve_duration
         stats["phase_fraction_max_abs_error_vs_fp64"] = err
-        stats["fp64_replayed"] = len(sample)
+        stats["fp64_replays"] = rows
         if capacity:
             stats["throughput_share_of_batched_capacity"] = \
                 stats["throughput_cell_updates_per_s"] / capacity
@@ -681,16 +684,33 @@ def main():
         sys.exit(2)
 
     if args.calibrate_copy:
+        # a plain device copy of the headline's four FP64 fields: the bytes
+        # per dispatch scale the rocprofv3 FETCH_SIZE / WRITE_SIZE counters,
+        # and its time (HIP events after the clock has settled) is the
+        # streaming ceiling the step kernels are compared with
         import torch
         n = 4098 * 4098
         src = [torch.ones(n, dtype=torch.float64, device="cuda") for _ in range(4)]
         dst = [torch.empty_like(x) for x in src]
-        for _ in range(25):
+        for _ in range(50):
             for a, b in zip(dst, src):
                 a.copy_(b)
-        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        reps = 50
+        e0.record()
+        for _ in range(reps):
+            for a, b in zip(dst, src):
+                a.copy_(b)
+        e1.record()
+        e1.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / (4 * reps)
+        moved = 2 * n * 8
         print(json.dumps({"calibration": "torch copy_ of 4 x %d FP64" % n,
-                          "bytes_read_per_dispatch": n * 8, "bytes_written_per_dispatch": n * 8}))
+                          "bytes_read_per_dispatch": n * 8, "bytes_written_per_dispatch": n * 8,
+                          "dispatches_timed": 4 * reps, "us_per_dispatch": us,
+                          "copy_GB_per_s": moved / us / 1e3,
+                          "copy_frac_of_peak": moved / (us * 1e-6) / HBM_PEAK}))
         return
 
     if args.kernels_only:
