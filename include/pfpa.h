@@ -150,6 +150,19 @@ int pf_summarize_f64(pf_ctx* ctx, int nx, int ny, const double* conc_Cr, const d
 int pf_summarize_f32(pf_ctx* ctx, int nx, int ny, const float* conc_Cr, const float* conc_Nb,
                      const float* phi_del, const float* phi_lav, double* out5, void* stream);
 
+/* pf_summarize_f64 without a context, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc); no counterpart in the reference.  It
+ * allocates nothing: the reduction partials live in a fixed 80 KiB device
+ * array the library holds on every GPU it runs on until the process ends
+ * (looked up on the first call on each device).  Same result, bit for bit,
+ * as pf_summarize_f64 (compositions as stored: no storage offset).  Grids up
+ * to nx * ny < 2^31 cells.  Calls that overlap on one GPU (other streams or
+ * threads) race on the shared array; calls in stream order do not. */
+int pf_summarize_f64_noalloc(int nx, int ny, const double* conc_Cr, const double* conc_Nb,
+                             const double* phi_del, const double* phi_lav, double* out5,
+                             void* stream);
+
 /* Free energy of the fields: src/alloy625.cpp:752-784 (gibbs) + :786-806
  * (maskedgradient) summed as in summarize_energy (:865-919), with the
  * per-cell densities times dV = dx*dy; compensated FP64 sums in a fixed order.
