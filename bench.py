@@ -339,8 +339,15 @@ def batched_section(args):
         plain_inputs.append((s, f))
     log("batched: %d instances, %.3g cells, setup %.1f s" % (len(specs), cells,
                                                             time.perf_counter() - t_gen))
-    best, table = autotune_batch(batch, launches=8)
-    _, rows, depth, ls, g = best
+    if args.batch_tile:
+        # a fixed configuration instead of the autotuner (counter runs)
+        rows, depth, ls, g = args.batch_tile
+        batch.set_tile(rows, depth)
+        batch.set_launch_steps(ls)
+        table = [(0.0, rows, depth, ls, g)]
+    else:
+        best, table = autotune_batch(batch, launches=8)
+        _, rows, depth, ls, g = best
     tuned = {"rows": rows, "depth": depth, "launch_steps": ls, "launches_per_graph": g,
              "candidates": len(table),
              "table": [{"rate": r[0], "rows": r[1], "depth": r[2], "launch_steps": r[3],
@@ -423,6 +430,49 @@ def batched_section(args):
         "phase_fraction_max_abs_error_vs_fp64": max(errs),
         "phase_fraction_sample": {"jobs": sample, "steps": [after[k] for k in sample]},
     }
+
+
+def mobility_section(args):
+    """The variable-mobility Cahn-Hilliard divergence (pf_mobility_divergence_f64)
+    at the headline's grid: 24 FP64 input arrays read and 2 written per cell,
+    208 B per cell, timed with HIP events over back-to-back launches after
+    warm-up launches."""
+    import torch
+
+    from pfpa import api
+    N = args.size
+    shape = (N + 2, N + 2)
+    g = torch.Generator(device="cuda").manual_seed(11)
+
+    def field(lo, hi):
+        return lo + (hi - lo) * torch.rand(shape, dtype=torch.float64, device="cuda", generator=g)
+
+    conc = [field(0.0, 0.5) for _ in range(6)]
+    mob_cr = [field(1e-21, 1e-20) for _ in range(8)]
+    mob_nb = [field(1e-21, 1e-20) for _ in range(8)]
+    pd, pl = field(0.0, 1.0), field(0.0, 1.0)
+    out = [torch.empty_like(pd), torch.empty_like(pd)]
+    dx = 0.25e-9
+    st = torch.cuda.current_stream()
+    for _ in range(20):
+        api.chemical_convolution(conc, mob_cr, mob_nb, pd, pl, dx, dx, *out)
+    reps = args.kernel_reps
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        api.chemical_convolution(conc, mob_cr, mob_nb, pd, pl, dx, dx, *out)
+    e1.record(st)
+    e1.synchronize()
+    secs = e0.elapsed_time(e1) / 1e3 / reps
+    cells = shape[0] * shape[1]
+    alg = 26 * 8 * cells
+    del conc, mob_cr, mob_nb, pd, pl, out
+    return {"kernel": "pf::mobility_divergence_kernel", "grid_padded": list(shape),
+            "kernel_avg_us": secs * 1e6, "algorithmic_bytes_per_launch": alg,
+            "algorithmic_bytes_per_cell": 208, "achieved": alg / secs / 1e9,
+            "frac": alg / secs / HBM_PEAK, "unit": "GB/s", "bound": "hbm",
+            "cells_per_s": cells / secs}
 
 
 def dropin_section(args):
@@ -641,12 +691,16 @@ def main():
     ap.add_argument("--cpu-one-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-only", action="store_true")
-    ap.add_argument("--only", choices=["dropin", "batched", "serving", "precision"],
+    ap.add_argument("--only", choices=["dropin", "batched", "serving", "precision", "mobility"],
                     default=None, help="run one of the other sections alone (no headline line)")
     ap.add_argument("--dropin-steps", type=int, default=900,
                     help="steps of the C++ drop-in replay (the driver's call sequence)")
     ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=5000)
+    ap.add_argument("--batch-tile", type=int, nargs=4, default=None,
+                    metavar=("ROWS", "DEPTH", "STEPS", "LPG"),
+                    help="batched form: this tile, steps per launch and launches per graph "
+                         "instead of the autotuner's")
     ap.add_argument("--loop-steps", type=int, default=20,
                     help="launches of each plain loop (two-step and one-step)")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[6.0, 16.0, 24.0])
@@ -740,7 +794,8 @@ def main():
         # one section alone (for profiling), one JSON line
         args.size = args.size or 4096
         fn = {"dropin": dropin_section, "batched": batched_section,
-              "serving": serving_section, "precision": precision_section}[args.only]
+              "serving": serving_section, "precision": precision_section,
+              "mobility": mobility_section}[args.only]
         print(json.dumps({args.only: fn(args)}))
         return
 
@@ -776,6 +831,8 @@ def main():
     if not args.plain_only:
         out["dropin"] = dropin_section(args)
         log("dropin:", json.dumps(out["dropin"]))
+        out["mobility_divergence"] = mobility_section(args)
+        log("mobility:", json.dumps(out["mobility_divergence"]))
         out["batched"] = batched_section(args)
         log("batched:", json.dumps({k: v for k, v in out["batched"].items()
                                     if k != "autotuned"}))
