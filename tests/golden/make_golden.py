@@ -94,6 +94,18 @@ def boundary(f):
     f[ny - 1, :] = f[ny - 2, :]
 
 
+def boundary_periodic(f):
+    """The same ghost ring filled by the periodic wrap that the 256^2 plain case
+    asks for (the reference itself has only the no-flux copy): ghost columns
+    and rows := the interior column / row at the opposite side, corners from
+    the diagonally opposite interior corner."""
+    ny, nx = f.shape
+    f[:, 0] = f[:, nx - 2]
+    f[:, nx - 1] = f[:, 1]
+    f[0, :] = f[ny - 2, :]
+    f[ny - 1, :] = f[1, :]
+
+
 def convolution(src, mask):
     """convolution_kernel interior outputs (src/discretization.cu:550-603),
     accumulated in mask order; ghost outputs are left as garbage (NaN)."""
@@ -109,8 +121,10 @@ def convolution(src, mask):
     return out
 
 
-def replay_step(R, old, P):
-    """One pass of src/main.cpp:289-305 with the reference's kernels."""
+def replay_step(R, old, P, boundary=boundary):
+    """One pass of src/main.cpp:289-305 with the reference's kernels;
+    `boundary` fills the ghost rings (the reference's copy, or the periodic
+    wrap)."""
     xCr, xNb, pd, pl = [a.copy() for a in old]
     ny, nx = xCr.shape
     for a in (xCr, xNb, pd, pl):
@@ -207,7 +221,14 @@ def step_vectors(R):
     for _ in range(nsteps):
         fields = replay_step(R, fields, P)
         hist.append(np.array(fields))
-    out = {"fields": np.array(hist), "dt": np.array([dt]), "DCr": np.array(DCr),
+    # the same start and kernels with periodic ghost rings
+    fields = [a.copy() for a in hist[0]]
+    hist_p = [np.array(fields)]
+    for _ in range(nsteps):
+        fields = replay_step(R, fields, P, boundary_periodic)
+        hist_p.append(np.array(fields))
+    out = {"fields": np.array(hist), "fields_periodic": np.array(hist_p),
+           "dt": np.array([dt]), "DCr": np.array(DCr),
            "DNb": np.array(DNb), "kappa": np.array(kappa), "omega": np.array(omega),
            "Lmob": np.array([L, L]), "alpha": np.array([1.07e11]), "mask": np.array(mask),
            "dx": np.array([dx])}

@@ -139,6 +139,34 @@ def test_step_golden_fixture(gpu, step_golden):
     sim.close()
 
 
+def test_step_golden_fixture_periodic(gpu, step_golden):
+    """GPU in periodic mode vs the literal replay with periodic ghost rings
+    (14 x 11, 4 steps): interior and ghost ring (the kernel writes each ghost
+    cell with the value of the interior cell it wraps to, corners included)."""
+    G = step_golden
+    F = G["fields_periodic"]
+    s = M.Setup(N=(F.shape[3] - 2, F.shape[2] - 2), dx=float(G["dx"][0]), dt=float(G["dt"][0]),
+                bc=1, DCr=list(G["DCr"]), DNb=list(G["DNb"]), Lmob=list(G["Lmob"]),
+                kappa=list(G["kappa"]), omega=list(G["omega"]), alpha=float(G["alpha"][0]),
+                mask=list(G["mask"]))
+    sim = Simulation(s, [F[0][k] for k in range(4)], steps_per_launch=1)
+    for step in range(1, F.shape[0]):
+        sim.step(1)
+        got = sim.fields()
+        for k in range(4):
+            assert mixed_err(got[k], F[step][k]) <= TOL, (step, k)
+            full = np.abs(got[k] - F[step][k])
+            assert full.max() <= TOL * max(np.abs(F[step][k]).max(), 1.0), (step, k)
+    sim.close()
+    # and all four steps in one three-step plus one one-step launch
+    sim = Simulation(s, [F[0][k] for k in range(4)])
+    sim.step(4)
+    got = sim.fields()
+    for k in range(4):
+        assert mixed_err(got[k], F[4][k]) <= TOL, k
+    sim.close()
+
+
 @pytest.mark.parametrize("bc", [1, 0])
 def test_parity_256_1000_steps(gpu, oracle, bc):
     """The 256^2 plain case (2 delta + 2 Laves particles in a noisy matrix),

@@ -96,6 +96,28 @@ def test_step_golden_literal_replay(oracle, step_golden):
             assert full.max() <= 1e-12 * max(sf, 1.0)
 
 
+def test_step_golden_periodic_replay(oracle, step_golden):
+    """Oracle in periodic mode vs the same literal replay with the ghost rings
+    filled by the periodic wrap instead of the reference's copy (14 x 11,
+    4 steps): pins the periodic index map, which the reference does not have,
+    by something other than the oracle's own boundary code."""
+    G = step_golden
+    F = G["fields_periodic"]
+    P = oracle.make_params(G["dt"][0], G["DCr"], G["DNb"], G["kappa"], G["omega"], G["Lmob"],
+                           G["alpha"][0], G["mask"])
+    cur = [F[0][k].copy() for k in range(4)]
+    for s in range(1, F.shape[0]):
+        cur = oracle.step(cur, P, 1)
+        for k in range(4):
+            ref = F[s][k][1:-1, 1:-1]
+            a = cur[k][1:-1, 1:-1]
+            sf = np.abs(ref).max()
+            rel = np.abs(a - ref) / np.maximum(np.maximum(np.abs(a), np.abs(ref)), sf)
+            assert rel.max() <= 1e-12, (s, k)
+    # the periodic run differs from the no-flux one (the boundary matters here)
+    assert np.abs(F[-1][2] - G["fields"][-1][2])[1:-1, 1:-1].max() > 1e-6
+
+
 # ------------------------------------------------------------------ closed forms
 
 
