@@ -314,7 +314,7 @@ def batched_section(args):
     import torch
 
     from pfpa._lib import PfStepItem, check, lib
-    from pfpa.batch import PF_STORAGE_F32, Batch, job_bytes, make_desc
+    from pfpa.batch import PF_STORAGE_F32, PF_STORAGE_F64, Batch, job_bytes, make_desc
     from pfpa.sim import Simulation
     from pfpa.workloads import (autotune_batch, batch_cell_updates, batch_specs, spec_fields,
                                 spec_setup, step_counts)
@@ -371,6 +371,28 @@ def batched_section(args):
     batch.status()
     batch.close()
 
+    # the same mix and configuration with FP64 storage: separates what the
+    # persistent batched launch gains (filling the chip with many small grids)
+    # from what FP32 storage gains (half the bytes)
+    b64 = Batch(PF_STORAGE_F64, pool_bytes=sum(job_bytes(PF_STORAGE_F64, sp["n"], sp["n"])
+                                                for sp in specs) + (64 << 20),
+                max_jobs=len(specs))
+    for sp, (s, f) in zip(specs, plain_inputs):
+        b64.add(make_desc(s, target, nucleation=True, seed=1000 + sp["index"],
+                          instance=sp["index"]), f)
+    b64.set_tile(rows, depth)
+    b64.set_launch_steps(ls)
+    b64.run(g, g)
+    b64.sync()
+    before64 = step_counts(b64)
+    launches64 = max(g, min(launches, 300) // g * g)
+    t0 = time.perf_counter()
+    b64.run(launches64, g)
+    b64.sync()
+    fp64_storage_rate = batch_cell_updates(b64, before64) / (time.perf_counter() - t0)
+    b64.close()
+    log("batched FP64 storage: %.4g cell-updates/s" % fp64_storage_rate)
+
     # loops of plain FP64 launches over the same instances, issued from C, at
     # three, two and one steps per launch; the fastest is the comparison
     ctx = ctypes.c_void_p()
@@ -426,6 +448,9 @@ def batched_section(args):
         "plain_loop_fp64_value": loop_rate, "plain_loop_fp64_rates": loop_rates,
         "plain_loop_launches": args.loop_steps,
         "speedup_vs_plain_loop": batched_rate / loop_rate,
+        "fp64_storage_value": fp64_storage_rate,
+        "fp64_storage_launches": launches64,
+        "speedup_fp64_storage_vs_plain_loop": fp64_storage_rate / loop_rate,
         "autotuned": tuned,
         "phase_fraction_max_abs_error_vs_fp64": max(errs),
         "phase_fraction_sample": {"jobs": sample, "steps": [after[k] for k in sample]},
