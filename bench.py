@@ -561,8 +561,9 @@ def serving_section(args, capacity=None):
                 log("serving %s %.1f/s: %d of %d requests done" % (order, rate, done[0],
                                                                    len(reqs)))
 
+        rows = args.serve_rows if args.serve_rows == "auto" else int(args.serve_rows)
         finished, stats = serve(reqs, pool_bytes=args.serve_pool_gb << 30,
-                                launches_per_graph=args.serve_lpg, rows=args.serve_rows,
+                                launches_per_graph=args.serve_lpg, rows=rows,
                                 depth=3, launch_steps=3, log=progress, order=order)
         # FP64 replays of the requests with the least, the median and the most
         # work (cells x steps)
@@ -733,8 +734,9 @@ def main():
     ap.add_argument("--serve-pool-gb", type=int, default=4)
     ap.add_argument("--serve-lpg", type=int, default=17,
                     help="step launches per captured graph in the serving loop")
-    ap.add_argument("--serve-rows", type=int, default=16,
-                    help="rows per strip task of the serving batch")
+    ap.add_argument("--serve-rows", default="auto",
+                    help="rows per strip task of the serving batch, or 'auto' (16 while few "
+                         "cells run, 64 under load)")
     ap.add_argument("--no-srw", action="store_true",
                     help="skip the shortest-remaining-work serving run")
     ap.add_argument("--precision-size", type=int, default=8192)

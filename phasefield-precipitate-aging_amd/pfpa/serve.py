@@ -81,7 +81,8 @@ def request_offset(r):
 
 def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_steps=3,
           log=None, storage=PF_STORAGE_F32, time_limit=None, read_fields=False,
-          offset=True, order="fifo", fill_cells=16 << 20, max_hold_s=2.0, max_bypass_s=1.0):
+          offset=True, order="fifo", fill_cells=16 << 20, max_hold_s=2.0, max_bypass_s=1.0,
+          auto_rows=(16, 64), auto_cells=(12 << 20, 24 << 20)):
     """Run the request stream; returns (finished requests, stats).  Each device
     round is one replay of a graph of `launches_per_graph` step launches, each
     advancing every running job by up to `launch_steps` steps (1, 2 or 3).
@@ -97,6 +98,12 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
     launch; the others are held for the round (pf_batch_hold), except that a
     job held for `max_hold_s` in a row runs from then on until it is done.
     Results do not depend on the order: a held job resumes where it stopped.
+
+    rows: the strip-task height, or "auto": `auto_rows[0]` (short tasks, more
+    parallelism for a few small jobs) while the running jobs hold fewer than
+    `auto_cells[1]` cells, `auto_rows[1]` (less halo recomputation, the batched
+    form's best throughput) from there until they drop below `auto_cells[0]`.
+    The tile never changes a job's results.
     In both orders a request that does not fit in the pool lets the ones
     behind it pass, but once it has waited `max_bypass_s` since it first did
     not fit, nothing behind it is admitted until it fits (a large request is
@@ -105,7 +112,9 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
         raise ValueError("order must be 'fifo' or 'srw'")
     import torch
     batch = Batch(storage, pool_bytes=pool_bytes, max_jobs=max(16, len(requests)))
-    batch.set_tile(rows, depth)
+    auto = rows == "auto"
+    cur_rows = auto_rows[0] if auto else int(rows)
+    batch.set_tile(cur_rows, depth)
     batch.set_launch_steps(launch_steps)
     ext = torch.cuda.ExternalStream(batch.stream())
     pending = list(requests)       # sorted by arrival
@@ -122,7 +131,7 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
     workers = ThreadPoolExecutor(max_workers=1)
     cell_updates = 0
     busy = 0.0                     # device time of the rounds (HIP events on the batch stream)
-    rounds = held_rounds = 0
+    rounds = held_rounds = tile_switches = 0
     blocked_since = {}             # rid -> time it first did not fit
     t0 = time.perf_counter()
 
@@ -178,6 +187,17 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
             running[jid] = r
             progress[jid] = 0
         waiting = still
+        if running and auto:
+            cells_running = sum(r.n * r.n for r in running.values())
+            want = cur_rows
+            if cur_rows == auto_rows[0] and cells_running >= auto_cells[1]:
+                want = auto_rows[1]
+            elif cur_rows == auto_rows[1] and cells_running < auto_cells[0]:
+                want = auto_rows[0]
+            if want != cur_rows:
+                batch.set_tile(want, depth)
+                cur_rows = want
+                tile_switches += 1
         if running:
             if order == "srw":
                 now = time.perf_counter() - t0
@@ -263,6 +283,8 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
         "device_busy_share": busy / wall if wall > 0 else 0.0,
         "rounds": rounds,
         "held_job_rounds": held_rounds,
+        "rows": rows,
+        "tile_switches": tile_switches,
         "latency_p50_s": float(np.percentile(lat, 50)),
         "latency_p99_s": float(np.percentile(lat, 99)),
         "latency_max_s": float(lat.max()),

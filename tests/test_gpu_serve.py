@@ -112,3 +112,22 @@ def test_serving_shortest_remaining_work_same_results():
         for k in range(4):
             assert np.array_equal(r.fields[k], by_f[r.rid].fields[k]), (r.rid, k)
         assert r.summary == by_f[r.rid].summary
+
+
+def test_serving_auto_tile_same_results():
+    """The adaptive strip height (short tasks while few cells run, tall ones
+    under load) switches the batch's tile between rounds and changes no bits."""
+    reqs = make_requests(rate=300.0, duration=1.0, seed=13, n_lo=64, n_hi=200, steps_lo=20,
+                         steps_hi=120, max_requests=10)
+    for r in reqs:
+        r.arrival = 0.0
+    pool = 1 << 30
+    fixed, _ = serve(reqs, pool, launches_per_graph=3, rows=16, read_fields=True)
+    auto, st = serve(reqs, pool, launches_per_graph=3, rows="auto", read_fields=True,
+                     auto_rows=(8, 48), auto_cells=(20000, 60000))
+    assert st["tile_switches"] >= 1 and st["completed"] == len(reqs)
+    by_f = {r.rid: r for r in fixed}
+    for r in auto:
+        for k in range(4):
+            assert np.array_equal(r.fields[k], by_f[r.rid].fields[k]), (r.rid, k)
+        assert r.summary == by_f[r.rid].summary
