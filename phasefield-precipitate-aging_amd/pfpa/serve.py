@@ -82,7 +82,7 @@ def request_offset(r):
 def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_steps=3,
           log=None, storage=PF_STORAGE_F32, time_limit=None, read_fields=False,
           offset=True, order="fifo", fill_cells=16 << 20, max_hold_s=2.0, max_bypass_s=1.0,
-          auto_rows=(16, 64), auto_cells=(12 << 20, 24 << 20)):
+          auto_rows=(16, 64), auto_cells=(12 << 20, 24 << 20), auto_graphs=(1, 3)):
     """Run the request stream; returns (finished requests, stats).  Each device
     round is one replay of a graph of `launches_per_graph` step launches, each
     advancing every running job by up to `launch_steps` steps (1, 2 or 3).
@@ -102,8 +102,11 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
     rows: the strip-task height, or "auto": `auto_rows[0]` (short tasks, more
     parallelism for a few small jobs) while the running jobs hold fewer than
     `auto_cells[1]` cells, `auto_rows[1]` (less halo recomputation, the batched
-    form's best throughput) from there until they drop below `auto_cells[0]`.
-    The tile never changes a job's results.
+    form's best throughput) from there until they drop below `auto_cells[0]`;
+    in the loaded state a round also replays the graph `auto_graphs[1]` times
+    instead of `auto_graphs[0]`, so the host's polling between rounds takes a
+    smaller share of the device's time.  The tile never changes a job's
+    results.
     In both orders a request that does not fit in the pool lets the ones
     behind it pass, but once it has waited `max_bypass_s` since it first did
     not fit, nothing behind it is admitted until it fits (a large request is
@@ -114,6 +117,7 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
     batch = Batch(storage, pool_bytes=pool_bytes, max_jobs=max(16, len(requests)))
     auto = rows == "auto"
     cur_rows = auto_rows[0] if auto else int(rows)
+    graphs = auto_graphs[0] if auto else 1     # graph replays per round
     batch.set_tile(cur_rows, depth)
     batch.set_launch_steps(launch_steps)
     ext = torch.cuda.ExternalStream(batch.stream())
@@ -197,6 +201,7 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
             if want != cur_rows:
                 batch.set_tile(want, depth)
                 cur_rows = want
+                graphs = auto_graphs[1] if want == auto_rows[1] else auto_graphs[0]
                 tile_switches += 1
         if running:
             if order == "srw":
@@ -223,7 +228,7 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record(ext)
-            batch.run(launches_per_graph, launches_per_graph)   # asynchronous
+            batch.run(graphs * launches_per_graph, launches_per_graph)   # asynchronous
             e1.record(ext)
             rounds += 1
             # start building the next admissions' fields
