@@ -284,6 +284,7 @@ struct pf_batch {
   double* d_partials = nullptr;
   double* d_out5 = nullptr;
   hipStream_t stream = nullptr;
+  hipStream_t upload = nullptr;  // admissions' field uploads (overlap the running launches)
   int grid_blocks = 0;
   int rows = 16, depth = 3;
   int launch_steps = 1;  // steps per launch (1, 2 or 3)
@@ -435,7 +436,8 @@ int pf_batch_create(pf_batch** out, int storage, size_t pool_bytes, int max_jobs
       (e = hipMalloc((void**)&b->d_hdr, sizeof(pf::BatchHeader))) != hipSuccess ||
       (e = hipMalloc((void**)&b->d_partials, pf::BATCH_RB * 10 * sizeof(double))) != hipSuccess ||
       (e = hipMalloc((void**)&b->d_out5, 8 * sizeof(double))) != hipSuccess ||
-      (e = hipStreamCreate(&b->stream)) != hipSuccess) {
+      (e = hipStreamCreate(&b->stream)) != hipSuccess ||
+      (e = hipStreamCreateWithFlags(&b->upload, hipStreamNonBlocking)) != hipSuccess) {
     pf_batch_destroy(b);
     return bhip(e, "batch allocation");
   }
@@ -465,6 +467,7 @@ void pf_batch_destroy(pf_batch* b) {
   if (b->d_partials) (void)hipFree(b->d_partials);
   if (b->d_out5) (void)hipFree(b->d_out5);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->upload) (void)hipStreamDestroy(b->upload);
   delete b;
 }
 
@@ -519,7 +522,11 @@ int pf_batch_add(pf_batch* b, const pf_job_desc* d, const void* xCr, const void*
       return bfail(PF_ERR_ARG, "nucleation scan radius >= 1 cell: domain too small for batch mode");
     if (d->nuc_every < 1) return bfail(PF_ERR_ARG, "nuc_every must be >= 1");
   }
-  if (int r = sync_jobs_from_device(b)) return r;
+  // No wait for the launches in flight: the new job's slot and pool region
+  // are unused by them (a slot or region is freed only by pf_batch_remove,
+  // which waits for the stream, and the tables the launches read change only
+  // at the next pf_batch_commit, which waits for the stream too), so its
+  // fields upload while they run.
   int slot = -1;
   for (size_t j = 0; j < b->jobs.size(); j++)
     if (!b->slot_live[j]) {
@@ -561,13 +568,13 @@ int pf_batch_add(pf_batch* b, const pf_job_desc* d, const void* xCr, const void*
   hipError_t e;
   for (int k = 0; k < 4; k++) {
     e = hipMemcpyAsync(base + (J.off[0] + k * J.plane) * b->elem, src[k], bytes,
-                       hipMemcpyHostToDevice, b->stream);
+                       hipMemcpyHostToDevice, b->upload);
     if (e != hipSuccess) {
       pool_free(b, off, n);
       return bhip(e, "upload job fields");
     }
   }
-  e = hipStreamSynchronize(b->stream);
+  e = hipStreamSynchronize(b->upload);
   if (e != hipSuccess) return bhip(e, "upload job fields");
   b->jobs[slot] = J;
   b->slot_live[slot] = 1;

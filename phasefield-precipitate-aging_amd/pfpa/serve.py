@@ -151,16 +151,12 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
             prepared[r.rid] = workers.submit(build, r)
         return prepared[r.rid]
 
-    while pending or waiting or running:
-        now = time.perf_counter() - t0
-        if time_limit and now > time_limit:
-            break
-        while pending and pending[0].arrival <= now:
-            waiting.append(pending.pop(0))
+    def admit(now):
+        """Admit waiting requests while they fit; a request whose fields are
+        still being built waits for a later round while the device has work."""
+        nonlocal waiting
         if order == "srw":
             waiting.sort(key=lambda q: (q.n * q.n * q.steps, q.arrival))
-        # admit; a request whose fields are still being built waits for a
-        # later round while the device has work
         still = []
         barrier = False            # a request passed over too long blocks the rest
         for r in waiting:
@@ -191,6 +187,19 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
             running[jid] = r
             progress[jid] = 0
         waiting = still
+
+    while pending or waiting or running:
+        now = time.perf_counter() - t0
+        if time_limit and now > time_limit:
+            break
+        while pending and pending[0].arrival <= now:
+            waiting.append(pending.pop(0))
+        # Under load (the auto tile's loaded state) admissions upload their
+        # fields while the round runs and join the next round; otherwise they
+        # join this one.
+        loaded = auto and cur_rows == auto_rows[1]
+        if not loaded or not running:
+            admit(now)
         if running and auto:
             cells_running = sum(r.n * r.n for r in running.values())
             want = cur_rows
@@ -231,8 +240,12 @@ def serve(requests, pool_bytes, launches_per_graph=17, rows=16, depth=3, launch_
             batch.run(graphs * launches_per_graph, launches_per_graph)   # asynchronous
             e1.record(ext)
             rounds += 1
-            # start building the next admissions' fields
             now = time.perf_counter() - t0
+            if loaded:
+                while pending and pending[0].arrival <= now:
+                    waiting.append(pending.pop(0))
+                admit(now)
+            # start building the next admissions' fields
             for r in waiting[:4] + [q for q in pending[:8] if q.arrival <= now + 0.5]:
                 prepare(r)
             states = batch.poll_all()     # waits for the round

@@ -289,3 +289,42 @@ def test_autotuner_leaves_a_consistent_batch(gpu):
         for k in range(4):
             assert np.array_equal(got[k].astype(np.float64), want[k]), (jid, k)
     b.close()
+
+
+def test_admission_while_launches_run(gpu):
+    """pf_batch_add does not wait for the launches in flight: a job admitted
+    while they run (into the slot and pool region a retired job freed) uploads
+    its fields beside them and joins at the next commit.  Both the running job
+    and the new one end with the same bits as each run alone."""
+    sa = M.make_setup(120, 90, bc=1)
+    sb = M.make_setup(140, 100, bc=0)
+    sc = M.make_setup(110, 96, bc=1)
+    fa, fb, fc = (rand_fields(120, 90, 1, 31), rand_fields(140, 100, 0, 32),
+                  rand_fields(110, 96, 1, 33))
+
+    def alone(s, f, steps, inst):
+        b = Batch(PF_STORAGE_F32, pool_bytes=64 << 20)
+        b.set_launch_steps(3)
+        j = b.add(make_desc(s, steps, instance=inst), f)
+        b.run(steps, 0)
+        out = b.read(j)
+        b.close()
+        return out
+
+    b = Batch(PF_STORAGE_F32, pool_bytes=64 << 20)
+    b.set_launch_steps(3)
+    ja = b.add(make_desc(sa, 9, instance=1), fa)
+    jb = b.add(make_desc(sb, 300, instance=2), fb)
+    b.run(5, 0)
+    assert b.poll(ja) == (9, False)
+    b.remove(ja)                     # frees slot and region
+    b.run(40, 4)                     # asynchronous: B runs
+    jc = b.add(make_desc(sc, 60, instance=3), fc)   # admitted while B's launches run
+    assert jc == ja                  # the freed slot is reused
+    b.run(200, 4)
+    assert b.poll(jb) == (300, False) and b.poll(jc) == (60, False)
+    got_b, got_c = b.read(jb), b.read(jc)
+    b.close()
+    for got, want in ((got_b, alone(sb, fb, 300, 2)), (got_c, alone(sc, fc, 60, 3))):
+        for k in range(4):
+            assert np.array_equal(got[k], want[k]), k
