@@ -767,3 +767,44 @@ def test_mobility_divergence_conserves_mass(oracle, bc):
         interior = o[1:-1, 1:-1]
         assert abs(interior.sum()) <= 1e-12 * np.abs(interior).sum()
         assert np.abs(interior).max() > 1.0        # not trivially zero
+
+
+def test_five_point_mask_anisotropic(oracle):
+    """five_point_Laplacian_stencil (src/mesh.cpp:187-196) with dx != dy,
+    applied as the reference's mask convolution: quadratics in x and in y give
+    the exact Laplacian 2 each, constants give 0 (a swapped dx / dy or a wrong
+    centre fails this)."""
+    dx, dy = 0.5, 0.25
+    m = np.array(oracle.five_point_mask(dx, dy)).reshape(3, 3)
+    X = np.arange(9)[None, :] * dx * np.ones((7, 1))
+    Y = np.arange(7)[:, None] * dy * np.ones((1, 9))
+
+    def conv(f):
+        out = np.zeros((5, 7))
+        for j in range(3):
+            for i in range(3):
+                out += m[j, i] * f[j:j + 5, i:i + 7]
+        return out
+
+    assert np.allclose(conv(X ** 2), 2.0, rtol=1e-12)
+    assert np.allclose(conv(Y ** 2), 2.0, rtol=1e-12)
+    assert np.allclose(conv(3.0 * X ** 2 - Y ** 2 + 0.7), 4.0, rtol=1e-12)
+    assert np.abs(conv(np.full((7, 9), 0.3))).max() <= 1e-12 * np.abs(m).max()
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 6])
+def test_run_equals_repeated_steps(oracle, n):
+    """orc_run's buffer ping-pong leaves the result of n steps in the first
+    buffer set, odd or even n: the same bits as n single steps."""
+    P = _params(oracle, 1e-7)
+    f = [np.ascontiguousarray(a) for a in
+         (np.full((9, 11), 0.3), np.full((9, 11), 0.02), np.zeros((9, 11)), np.zeros((9, 11)))]
+    rng = np.random.default_rng(n)
+    f[0] = f[0] + 1e-3 * rng.standard_normal(f[0].shape)
+    f[2][3:6, 4:8] = 0.8
+    cur = [a.copy() for a in f]
+    for _ in range(n):
+        cur = oracle.step(cur, P, 1)
+    got = oracle.run(f, P, 1, n)
+    for a, b in zip(got, cur):
+        assert np.array_equal(a, b)
