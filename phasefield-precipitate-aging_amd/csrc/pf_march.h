@@ -42,10 +42,22 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   return x * per + min(x, rem) + k;
 }
 
-// interior index map for i in [-1, n] (values beyond are clamped, unused)
+// Interior index map.  No-flux (BC 0): even reflection about the domain
+// faces, period 2n (-1 -> 0, -2 -> 1, n -> n-1, n+1 -> n-2, ...).  At the one
+// ghost layer (i = -1, n) this is the reference's ghost copy
+// (src/discretization.cu:360-394); deeper halo cells (the multi-step march)
+// are the mirror images that make every intermediate level's halo equal,
+// bit for bit, to the ghost copy of that level (see marchk_impl).  Any i maps
+// into [0, n).  Periodic (BC 1): wrap, for i in [-n, 2n) (values beyond are
+// clamped, unused).
 template <int BC>
 __device__ __forceinline__ int bc_map(int i, int n) {
-  if (BC == 0) return i < 0 ? 0 : (i >= n ? n - 1 : i);
+  if (BC == 0) {
+    const int p = 2 * n;
+    int m = i % p;
+    if (m < 0) m += p;
+    return m < n ? m : p - 1 - m;
+  }
   if (i < 0) return i + n;
   if (i >= n) return (i - n) < n ? i - n : n - 1;
   return i;
@@ -459,20 +471,21 @@ __device__ __forceinline__ void march(
 // The intermediate levels are exactly what one-step launches would have
 // stored: values are rounded to the storage type before the next level uses
 // them, and values at positions outside the interior are the images the
-// one-step kernel writes into the ghost ring.  Periodic images come out of the
-// index map directly (a halo lane at gx = -1 reads the neighbourhood of column
-// Nx-1).  No-flux images are copies of the adjacent interior cell's value at
-// the same level: taken from the neighbouring lane in x, and from the adjacent
-// row of the level's window in y.  So K steps here give the same result as K
-// one-step launches.  Symmetric five-point masks, no nucleation (the caller runs
-// nucleation steps one at a time).
-//
-// IMG: whether this wave's strip and chunk can meet a no-flux image (an edge
-// strip, or a chunk within K rows of the top or bottom).  marchk() below
-// picks the variant per wave, so the ~95% of waves that meet none carry none
-// of the image handling (which the compiler would otherwise execute on every
-// wave as selects).
-template <typename T, int BC, int K, int DEPTH, int SW, bool IMG>
+// one-step kernel writes into the ghost ring.  Both come out of the index map
+// of the old fields (bc_map):
+//  - periodic: a halo lane at gx = -1 reads the neighbourhood of column Nx-1;
+//  - no-flux: the old fields are read through the even reflection (column -1
+//    is column 0, -2 is 1, ...).  The step is pointwise apart from the
+//    five-point Laplacian, and the Laplacian of a cell and of its mirror
+//    image add the same two values in x (W + E = u[1] + u[0] and u[0] + u[1])
+//    and in y, in the same order otherwise; floating-point addition being
+//    commutative, every level's value at -1 is its value at 0 bit for bit,
+//    which is the ghost copy a one-step launch makes.  So no image needs a
+//    fix-up.
+// So K steps here give the same result as K one-step launches.  Symmetric
+// five-point masks, no nucleation (the caller runs nucleation steps one at a
+// time).
+template <typename T, int BC, int K, int DEPTH, int SW>
 __device__ __forceinline__ void marchk_impl(
     const T* __restrict__ iCr, const T* __restrict__ iNb, const T* __restrict__ ipd,
     const T* __restrict__ ipl, T* __restrict__ oCr, T* __restrict__ oNb, T* __restrict__ opd,
@@ -488,8 +501,6 @@ __device__ __forceinline__ void marchk_impl(
   const int gx = strip * SW + lane - K;  // interior column of this lane (may be a halo)
   const int px = 1 + bc_map<BC>(gx, Nx);
   const bool out_lane = (lane >= K) && (lane < K + SW) && (gx < Nx);
-  // does this wave hold a no-flux image column (gx = -1 or gx = Nx)?
-  const bool x_images = IMG && (strip == 0 || strip * SW + WAVE - 1 - K >= Nx);
 
   const int y0 = chunk * rows;
   const int y1 = min(y0 + rows, Ny);
@@ -511,8 +522,7 @@ __device__ __forceinline__ void marchk_impl(
   Raw<T> ring[DEPTH];
 
   // One step at the centre of level `lv`'s window (slots behind / centre /
-  // ahead): the next level's values, rounded to storage, with no-flux x
-  // images, derived into `v`, `c`.
+  // ahead): the next level's values, rounded to storage, derived into `v`, `c`.
   auto level_up = [&](const Point& Sth, const Point& C, const Centre& cc, const Point& Nth,
                       Point& v, Centre& c) {
     const Point W = shift_left(C), E = shift_right(C);
@@ -525,43 +535,17 @@ __device__ __forceinline__ void marchk_impl(
     r.nb = (T)n1;
     r.pd = (T)n2;
     r.pl = (T)n3;
-    if (x_images) {
-      const bool lo = gx == -1, hi = gx == Nx;
-      const double c0 = (double)r.cr, c1 = (double)r.nb, c2 = (double)r.pd, c3 = (double)r.pl;
-      const double fr0 = from_right(c0), fr1 = from_right(c1), fr2 = from_right(c2),
-                   fr3 = from_right(c3);
-      const double fl0 = from_left(c0), fl1 = from_left(c1), fl2 = from_left(c2),
-                   fl3 = from_left(c3);
-      r.cr = (T)(lo ? fr0 : (hi ? fl0 : c0));
-      r.nb = (T)(lo ? fr1 : (hi ? fl1 : c1));
-      r.pd = (T)(lo ? fr2 : (hi ? fl2 : c2));
-      r.pl = (T)(lo ? fr3 : (hi ? fl3 : c3));
-    }
     derive<T>(P, r, v, c);
   };
 
-  // Levels 1..K-1 at iteration t (slots are compile-time in both callers).
-  // Level l computes row y0-K+t-l; a no-flux image row (-1 or Ny) becomes a
-  // copy of the adjacent interior row of the same level: row -1 when row 0
-  // arrives one iteration later (always within the prologue, `top`), row Ny
-  // from row Ny-1 at once.
-  auto middle_levels = [&](int t, int sa, int sc, int sb, int lmax_active, bool top) {
+  // Levels 1..K-1 at iteration t (slots are compile-time in both callers):
+  // level l computes row y0-K+t-l.
+  auto middle_levels = [&](int sa, int sc, int sb, int lmax_active) {
 #pragma unroll
     for (int l = 1; l < K; l++) {
       if (l > lmax_active) break;
       level_up(win[l - 1][sb], win[l - 1][sc], cen[l - 1][sc], win[l - 1][sa], win[l][sa],
                cen[l][sa]);
-      if (IMG) {
-        const int row = y0 - K + t - l;
-        if (top && row == 0) {  // the image row -1 (when this chunk has one) is in slot sc
-          win[l][sc] = win[l][sa];
-          cen[l][sc] = cen[l][sa];
-        }
-        if (row == Ny) {
-          win[l][sa] = win[l][sc];
-          cen[l][sa] = cen[l][sc];
-        }
-      }
     }
   };
 
@@ -578,7 +562,7 @@ __device__ __forceinline__ void marchk_impl(
         for (int j = 0; j < DEPTH; j++) load(ring[j], min(y0 + K + j, y_last));
       }
       // level l is active from iteration 2l on
-      middle_levels(t, sa, sc, sb, t / 2, true);
+      middle_levels(sa, sc, sb, t / 2);
     }
   }
 
@@ -588,12 +572,11 @@ __device__ __forceinline__ void marchk_impl(
     for (int k = 0; k < UNROLL; k++) {
       if (ib + k >= nrows) return;
       const int y = y0 + ib + k;
-      const int t = 2 * K + ib + k;
       const int sa = (2 * K + k) % 3, sc = (2 * K + k + 2) % 3, sb = (2 * K + k + 1) % 3;
       const int rs = k % DEPTH;
       derive<T>(P, ring[rs], win[0][sa], cen[0][sa]);  // old row y + K
       load(ring[rs], min(y + K + DEPTH, y_last));
-      middle_levels(t, sa, sc, sb, K - 1, false);
+      middle_levels(sa, sc, sb, K - 1);
       // level K: row y from level K-1's rows y-1 (behind), y, y+1 (ahead)
       const Point& Sth = win[K - 1][sb];
       const Point& C = win[K - 1][sc];
@@ -617,19 +600,8 @@ __device__ __forceinline__ void marchk(
     const T* __restrict__ ipl, T* __restrict__ oCr, T* __restrict__ oNb, T* __restrict__ opd,
     T* __restrict__ opl, int Nx, int Ny, int pitch, int strip, int chunk, int rows,
     const StepParams& P) {
-  if (BC == 0) {
-    // old rows read: y0-K .. y1-1+K; lanes cover columns strip*SW-K .. strip*SW+63-K
-    const int y0 = chunk * rows, y1 = min(y0 + rows, Ny);
-    const bool edge = strip == 0 || strip * SW + WAVE - 1 - K >= Nx || y0 - K < 0 ||
-                      y1 - 1 + K >= Ny;
-    if (edge) {
-      marchk_impl<T, BC, K, DEPTH, SW, true>(iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny,
-                                             pitch, strip, chunk, rows, P);
-      return;
-    }
-  }
-  marchk_impl<T, BC, K, DEPTH, SW, false>(iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny, pitch,
-                                          strip, chunk, rows, P);
+  marchk_impl<T, BC, K, DEPTH, SW>(iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny, pitch, strip,
+                                   chunk, rows, P);
 }
 
 }  // namespace pf
