@@ -13,9 +13,10 @@
 // several rows of HBM requests in flight; the row loop is unrolled by 6 so the
 // window and the ring rotate by register renaming instead of moves.
 //
-// Boundary conditions are index maps (no ghost reads): no-flux clamps to the
-// nearest interior cell, which is exactly the reference's ghost copy
-// (src/discretization.cu:360-394) for every interior output; periodic wraps.
+// Boundary conditions are index maps (no ghost reads): no-flux reflects about
+// the domain faces, which at the one ghost layer the one-step march reads is
+// exactly the reference's ghost copy (src/discretization.cu:360-394); periodic
+// wraps.
 // Ghost cells of the outputs are written with the value of the cell they
 // mirror.  Storage T is double or float; all arithmetic is FP64.
 #pragma once
