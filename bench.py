@@ -380,8 +380,16 @@ def batched_section(args):
     for sp, (s, f) in zip(specs, plain_inputs):
         b64.add(make_desc(s, target, nucleation=True, seed=1000 + sp["index"],
                           instance=sp["index"]), f)
-    b64.set_tile(rows, depth)
-    b64.set_launch_steps(ls)
+    # its own strip height (FP64 storage moves twice the bytes and prefers
+    # shorter chunks than FP32), the FP32 winner's depth, steps and graphs
+    if args.batch_tile:
+        rows64 = rows
+        b64.set_tile(rows, depth)
+        b64.set_launch_steps(ls)
+    else:
+        best64, _ = autotune_batch(b64, launches=8, rows_set=(32, 48, 64), depth_set=(depth,),
+                                   launch_steps_set=(ls,), graph_set=(g,))
+        rows64 = best64[1]
     b64.run(g, g)
     b64.sync()
     before64 = step_counts(b64)
@@ -391,7 +399,7 @@ def batched_section(args):
     b64.sync()
     fp64_storage_rate = batch_cell_updates(b64, before64) / (time.perf_counter() - t0)
     b64.close()
-    log("batched FP64 storage: %.4g cell-updates/s" % fp64_storage_rate)
+    log("batched FP64 storage: %.4g cell-updates/s (rows %d)" % (fp64_storage_rate, rows64))
 
     # loops of plain FP64 launches over the same instances, issued from C, at
     # three, two and one steps per launch; the fastest is the comparison
@@ -450,6 +458,7 @@ def batched_section(args):
         "speedup_vs_plain_loop": batched_rate / loop_rate,
         "fp64_storage_value": fp64_storage_rate,
         "fp64_storage_launches": launches64,
+        "fp64_storage_rows": rows64,
         "speedup_fp64_storage_vs_plain_loop": fp64_storage_rate / loop_rate,
         "autotuned": tuned,
         "phase_fraction_max_abs_error_vs_fp64": max(errs),
