@@ -71,6 +71,7 @@ __host__ __device__ __forceinline__ bool nuc_due(const BatchJob& J) {
 constexpr int BATCH_RB = 16;  // reduction blocks per job
 
 // persistent distribution: XCD x owns the contiguous wave range [x*T/8, (x+1)*T/8)
+// = [first, last); stride = the XCD's wave count, start = this wave's slot in it
 __device__ __forceinline__ void xcd_range(int total, int& first, int& last, int& stride,
                                           int& start) {
   const int x = blockIdx.x & 7;
@@ -83,24 +84,21 @@ __device__ __forceinline__ void xcd_range(int total, int& first, int& last, int&
 }
 
 // XCD x (the hardware places block b on XCD b % 8) works through the
-// contiguous task range [x*T/8, (x+1)*T/8) in order; a wave claims its next
-// task from the XCD's counter (one atomic, by lane 0), so waves that draw
-// short tasks take more of them.  The claim for the next task is issued
-// before the current task runs and read after it.
+// contiguous task range [x*T/8, (x+1)*T/8) in order.  Each of the XCD's
+// waves takes one of the range's first tasks directly; after that a wave
+// claims its next task from the XCD's counter (one atomic, by lane 0), so
+// waves that draw short tasks take more of them.  The claim for the next
+// task is issued before the current task runs and read after it; waves with
+// no first task exit without touching the counter.
 template <typename T, int DEPTH>
 __global__ __launch_bounds__(WAVE * WPB) void batch_step_kernel(const BatchJob* __restrict__ jobs,
                                                                 const int* __restrict__ wave_job,
                                                                 BatchHeader* __restrict__ hdr,
                                                                 T* __restrict__ pool) {
-  const int x = blockIdx.x & 7;
-  const int total = hdr->total_waves;
-  const int first = (int)(((long)total * x) >> 3);
-  const int last = (int)(((long)total * (x + 1)) >> 3);
-  unsigned int* ctr = &hdr->next_task[x][0];
+  int first, last, nwaves, w;
+  xcd_range(hdr->total_waves, first, last, nwaves, w);
+  unsigned int* ctr = &hdr->next_task[blockIdx.x & 7][0];
   const bool lane0 = (threadIdx.x & (WAVE - 1)) == 0;
-  unsigned int claim = 0;
-  if (lane0) claim = atomicAdd(ctr, 1u);
-  int w = first + (int)__builtin_amdgcn_readfirstlane(claim);
   const NucParams N{};
   const Scratch S{};
   while (w < last) {
@@ -109,7 +107,7 @@ __global__ __launch_bounds__(WAVE * WPB) void batch_step_kernel(const BatchJob* 
     const int j = __builtin_amdgcn_readfirstlane(wave_job[w]);
     const BatchJob& J = jobs[j];
     if (!J.active) {
-      w = first + (int)__builtin_amdgcn_readfirstlane(next);
+      w = first + nwaves + (int)__builtin_amdgcn_readfirstlane(next);
       continue;
     }
     const int local = w - J.wave0;
@@ -137,7 +135,7 @@ __global__ __launch_bounds__(WAVE * WPB) void batch_step_kernel(const BatchJob* 
       PF_BATCH_MARCH(1);
     }
 #undef PF_BATCH_MARCH
-    w = first + (int)__builtin_amdgcn_readfirstlane(next);
+    w = first + nwaves + (int)__builtin_amdgcn_readfirstlane(next);
   }
 }
 
