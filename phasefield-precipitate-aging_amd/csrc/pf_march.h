@@ -550,18 +550,19 @@ __device__ __forceinline__ void marchk_impl(
     }
   };
 
-  // ---- prologue: iterations t = 0 .. 2K-1 (level K not yet active)
+  // ---- prologue: iterations t = 0 .. 2K-1 (level K not yet active).  Every
+  // row the prologue reads, and the ring, is requested before the first is
+  // used, so a wave waits for one memory latency at its start, not 2K.
   {
+    Raw<T> pro[2 * K];
+#pragma unroll
+    for (int t = 0; t < 2 * K; t++) load(pro[t], y0 - K + t);
+#pragma unroll
+    for (int j = 0; j < DEPTH; j++) load(ring[j], min(y0 + K + j, y_last));
 #pragma unroll
     for (int t = 0; t < 2 * K; t++) {
       const int sa = t % 3, sc = (t + 2) % 3, sb = (t + 1) % 3;
-      Raw<T> r;
-      load(r, y0 - K + t);
-      derive<T>(P, r, win[0][sa], cen[0][sa]);
-      if (t == 2) {  // start the ring once the first window is full
-#pragma unroll
-        for (int j = 0; j < DEPTH; j++) load(ring[j], min(y0 + K + j, y_last));
-      }
+      derive<T>(P, pro[t], win[0][sa], cen[0][sa]);
       // level l is active from iteration 2l on
       middle_levels(sa, sc, sb, t / 2);
     }
