@@ -6,7 +6,7 @@ import ctypes
 import numpy as np
 
 from ._lib import PfError, PfParams, check, lib
-from .model import IFCE_WIDTH, LATTICE_CONST, SIGMA
+from .model import LATTICE_CONST, SIGMA
 
 PF_STORAGE_F64 = 0
 PF_STORAGE_F32 = 1

@@ -17,7 +17,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import model as M
-from .batch import PF_STORAGE_F32, Batch, job_bytes, make_desc
+from .batch import PF_STORAGE_F32, Batch, make_desc
 
 
 @dataclass
