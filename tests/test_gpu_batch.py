@@ -328,3 +328,31 @@ def test_admission_while_launches_run(gpu):
     for got, want in ((got_b, alone(sb, fb, 300, 2)), (got_c, alone(sc, fc, 60, 3))):
         for k in range(4):
             assert np.array_equal(got[k], want[k]), k
+
+
+def test_task_claims_with_retired_jobs(gpu):
+    """More strip tasks than the persistent grid has waves (each wave takes
+    one task directly, then claims more from its XCD's counter), with half
+    the jobs retiring part-way: the other jobs, and a retired one, are the
+    same bits as the plain FP32 path run alone."""
+    n, count = 254, 40
+    tasks_per_job = ((n + 57) // 58) * ((n + 7) // 8)      # 58-column strips, 8-row chunks
+    assert count * tasks_per_job > 256 * 4 * 4             # more tasks than any grid's waves
+    b = Batch(PF_STORAGE_F32, pool_bytes=512 << 20)
+    b.set_tile(8, 3)
+    b.set_launch_steps(3)
+    jobs = []
+    for k in range(count):
+        s = M.make_setup(n, n, bc=k % 2)
+        f = rand_fields(n, n, k % 2, 500 + k)
+        steps = 7 if k % 2 else 20                          # odd jobs retire after 3 launches
+        jobs.append((b.add(make_desc(s, steps, instance=k), f), s, f, steps))
+    b.run(8, 4)                                             # graph replays after the retirements
+    for k in (0, 1, count - 2):
+        jid, s, f, steps = jobs[k]
+        assert b.poll(jid) == (steps, False)
+        got = b.read(jid)
+        want = plain(s, f, steps, torch.float32)
+        for q in range(4):
+            assert np.array_equal(got[q].astype(np.float64), want[q]), (k, q)
+    b.close()
