@@ -5,23 +5,23 @@ dx = 0.25 nm, periodic, seeded uniform matrix composition in
 Ni-(0.0202..0.0269)Nb-(0.2794..0.3288)Cr with a central enriched strip (1/8 of
 the width) at Ni-0.169Nb-0.272Cr, nucleation on (Philox seed 42, every
 floor(1e-4/dt) steps, fused into the step), dt = 0.1 dx^2 / (4 D_max).
-A "step" is one explicit Euler step over the whole grid; steps run three per
-launch (the temporally blocked kernel) except around nucleation steps, which
-run one per launch with the nucleation pass fused.  value = interior cells x
+A "step" is one explicit Euler step over the whole grid; steps run four per
+launch (the temporally blocked kernel; FP32 storage: three) except around
+nucleation steps, which run one per launch with the nucleation pass fused.  value = interior cells x
 timed steps / time of the timed loop (HIP events), fields resident in HBM.
 
 Clock state: the GPU's clock takes ~50 launches of sustained load to settle
 (the first launches of a cold run are up to 30% slower), so before the warm-up
-steps the three-step kernel runs --settle launches on scratch copies of the
+steps the four-step kernel runs --settle launches on scratch copies of the
 fields; the timed loop and the kernel leg then run in the same settled state,
 and the line fails (no output, exit 3) if the kernel leg's time per step
 exceeds the timed loop's.
 
-roofline: the dominant kernel is the three-step kernel; its algorithmic bytes
-per launch (FP64: 4 fields read + 4 written = 64 B per cell, for three
+roofline: the dominant kernel is the four-step kernel; its algorithmic bytes
+per launch (FP64: 4 fields read + 4 written = 64 B per cell, for four
 cell-updates) over its average launch duration (HIP events on the launch
 stream, --kernel-reps back-to-back launches right after the timed loop); the
-two-step and one-step kernels are reported beside it.  traffic: HBM bytes per
+three-, two- and one-step kernels are reported beside it.  traffic: HBM bytes per
 launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs under profiles/ (scaled
 by a copy kernel).
 cpu_baseline: the FP64 CPU oracle built -O3 -march=native on the host
@@ -32,13 +32,14 @@ core, and the 256^2 plain case (1000 steps) in full.
 Also reported (same JSON line):
   dropin    -- the reference driver's per-step call sequence through the C++
                drop-in API (tests/cpp/replay_main) on the same grid (no-flux,
-               nucleation on, no I/O), grouped three steps per launch and one.
+               nucleation on, no I/O), grouped four steps per launch (eight
+               calls to two launches), three, and one.
   batched   -- 512 independent simulations, square sides log-uniform in
                128..1024, FP32 storage with FP64 arithmetic and reductions, one
                persistent launch per one, two or three steps replayed from a
                hipGraph; tile, steps per launch and launches per graph from the
-               autotuner; against loops of plain FP64 launches (three-, two-
-               and one-step) over the same instances; phase-fraction error
+               autotuner; against loops of plain FP64 launches (four-, three-,
+               two- and one-step) over the same instances; phase-fraction error
                against FP64 replays of sampled instances.
   precision -- 8192^2, the headline's initialisation, FP32 storage (with and
                without a composition offset) against FP64, 20,000 steps: max
@@ -213,9 +214,11 @@ def plain_section(args, dtype_name="f64"):
     with torch.cuda.stream(stream):
         a = [x.clone() for x in sim.a]
         b = [torch.empty_like(x) for x in a]
-    fn = {3: lib().pf_step3_f64 if f64 else lib().pf_step3_f32,
+    fn = {4: lib().pf_step4_f64 if f64 else lib().pf_step4_f32,
+          3: lib().pf_step3_f64 if f64 else lib().pf_step3_f32,
           2: lib().pf_step2_f64 if f64 else lib().pf_step2_f32,
           1: lib().pf_step_f64 if f64 else lib().pf_step_f32}
+    K = sim.steps_per_launch  # the dominant kernel: four steps per launch (FP64), three (FP32)
 
     def launch(k, n):
         for i in range(n):
@@ -235,7 +238,7 @@ def plain_section(args, dtype_name="f64"):
 
     # settle the clock on the scratch copies, then warm up and time the
     # simulation's own steps
-    settle_s = time_kernel(3, args.settle) if args.settle > 0 else 0.0
+    settle_s = time_kernel(K, args.settle) if args.settle > 0 else 0.0
     sim.step(args.warmup)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -248,17 +251,16 @@ def plain_section(args, dtype_name="f64"):
     value = cells * args.steps / (ms / 1e3)
 
     # The kernels alone, back to back on the sim's stream right after the
-    # timed loop (same clock state): the three-step kernel (the dominant
-    # one), the two-step kernel (what the schedule leaves between nucleation
-    # steps) and the one-step kernel (nucleation steps; the drop-in API with
-    # grouping off).
-    kern3_s = time_kernel(3, args.kernel_reps)
-    kern2_s = time_kernel(2, args.kernel_reps)
-    kern1_s = time_kernel(1, args.kernel_reps)
-    if kern3_s / 3 * 1e3 > ms / args.steps:
-        log("bench.py: the three-step kernel's time per step (%.2f us, %d back-to-back "
+    # timed loop (same clock state): the K-step kernel (the dominant one),
+    # the shorter multi-step kernels (what the schedule leaves between
+    # nucleation steps) and the one-step kernel (nucleation steps; the drop-in
+    # API with grouping off).
+    kern_s = {k: time_kernel(k, args.kernel_reps) for k in range(K, 0, -1)}
+    kernK_s = kern_s[K]
+    if kernK_s / K * 1e3 > ms / args.steps:
+        log("bench.py: the %d-step kernel's time per step (%.2f us, %d back-to-back "
             "launches) exceeds the timed loop's (%.2f us): the two measurements are not in "
-            "the same clock state" % (kern3_s / 3 * 1e6, args.kernel_reps,
+            "the same clock state" % (K, kernK_s / K * 1e6, args.kernel_reps,
                                       ms / args.steps * 1e3))
         sys.exit(3)
     bpc = 64 if f64 else 32
@@ -273,38 +275,43 @@ def plain_section(args, dtype_name="f64"):
 
     nuc_every = sim.nuc_every
     sim.close()
-    valu_us = load_valu_issue("pf::stepk_kernel<%s, 1, 3, 3>" % ("double" if f64 else "float")) \
-        if N == 4096 else None
+    tname = "double" if f64 else "float"
+    kname = {4: "pf::step4_kernel<%s, 1, 3>" % tname, 3: "pf::stepk_kernel<%s, 1, 3, 3>" % tname,
+             2: "pf::stepk_kernel<%s, 1, 2, 3>" % tname}
+    valu_us = load_valu_issue(kname[K]) if N == 4096 else None
+    words = {1: "one", 2: "two", 3: "three", 4: "four"}
+    others = {"%s_step_kernel" % words[k]:
+              other("%s (%s fused step%s per launch)" % (kname.get(k, "pf::step_kernel"),
+                                                       words[k], "s" if k > 1 else ""), k,
+                    kern_s[k]) for k in range(K - 1, 0, -1)}
     return {
         "value": value, "ms_per_step": ms / args.steps, "N": N, "nuc_every": nuc_every,
         "dt": s.dt,
         "timing": {
-            "clock_settle": "%d back-to-back three-step launches on scratch copies of the "
+            "clock_settle": "%d back-to-back %s-step launches on scratch copies of the "
                             "fields before the warm-up (%.1f us each)"
-                            % (args.settle, settle_s * 1e6),
+                            % (args.settle, words[K], settle_s * 1e6),
             "kernel_leg": "%d back-to-back launches of each kernel on the scratch copies right "
                           "after the timed loop" % args.kernel_reps,
-            "three_step_kernel_us_per_step": kern3_s / 3 * 1e6,
+            "%s_step_kernel_us_per_step" % words[K]: kernK_s / K * 1e6,
             "timed_loop_us_per_step": ms / args.steps * 1e3,
         },
         "roofline": {
-            "bound": "hbm", "achieved": alg_bytes / kern3_s / 1e9, "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": alg_bytes / kern3_s / HBM_PEAK,
-            "traffic": load_traffic("plain3_%s_%d" % (tag, N)),
-            "kernel": "pf::stepk_kernel<K=3> (three fused steps per launch)",
-            "kernel_avg_us": kern3_s * 1e6,
+            "bound": "hbm", "achieved": alg_bytes / kernK_s / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": alg_bytes / kernK_s / HBM_PEAK,
+            "traffic": load_traffic("plain%d_%s_%d" % (K, tag, N)),
+            "kernel": "%s (%s fused steps per launch)" % (kname[K], words[K]),
+            "kernel_avg_us": kernK_s * 1e6,
             "algorithmic_bytes_per_launch": alg_bytes,
-            "cell_updates_per_launch": 3 * cells,
-            "algorithmic_bytes_per_cell_update": bpc / 3,
+            "cell_updates_per_launch": K * cells,
+            "algorithmic_bytes_per_cell_update": bpc / K,
             # the rate per step against the one-step roofline (bpc bytes per
             # cell-update at peak HBM), and the share of the launch the VALU
             # instructions alone need (rocprofv3 SQ_INSTS_VALU, profiles/)
             "cell_update_rate_vs_one_step_roofline": value * bpc / HBM_PEAK,
             "valu_issue_us": valu_us,
-            "valu_issue_frac": (valu_us / (kern3_s * 1e6)) if valu_us else None,
-            "two_step_kernel": other("pf::stepk_kernel<K=2> (two fused steps per launch)", 2,
-                                     kern2_s),
-            "one_step_kernel": other("pf::step_kernel (one fused step per launch)", 1, kern1_s),
+            "valu_issue_frac": (valu_us / (kernK_s * 1e6)) if valu_us else None,
+            **others,
         },
     }
 
@@ -402,7 +409,7 @@ def batched_section(args):
     log("batched FP64 storage: %.4g cell-updates/s (rows %d)" % (fp64_storage_rate, rows64))
 
     # loops of plain FP64 launches over the same instances, issued from C, at
-    # three, two and one steps per launch; the fastest is the comparison
+    # four, three, two and one steps per launch; the fastest is the comparison
     ctx = ctypes.c_void_p()
     check(lib().pf_ctx_create(ctypes.byref(ctx)), "ctx")
     bufs = []
@@ -424,7 +431,8 @@ def batched_section(args):
                 it.new_fields[k] = dst[k].data_ptr()
     st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     loop_rates = {}
-    for name, fn, per in (("three_step", lib().pf_step3_loop_f64, 3),
+    for name, fn, per in (("four_step", lib().pf_step4_loop_f64, 4),
+                          ("three_step", lib().pf_step3_loop_f64, 3),
                           ("two_step", lib().pf_step2_loop_f64, 2),
                           ("one_step", lib().pf_step_loop_f64, 1)):
         for par in range(2):
@@ -537,7 +545,7 @@ def dropin_section(args):
                 fh.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
         del f0
         nuc = M.nuc_interval(s.dt)
-        for spl in (3, 1):
+        for spl in (4, 3, 1):
             r = subprocess.run([harness, fin, os.path.join(td, "out.bin"), str(args.dropin_steps),
                                 repr(s.dt), repr(s.dx), str(nuc), "42", "0", str(spl)],
                                capture_output=True, text=True, timeout=600)
@@ -716,7 +724,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=600)
     # the GPU's clock takes ~50 launches (~30 ms) of sustained load to settle:
-    # three-step launches on scratch copies of the fields before the warm-up
+    # four-step launches on scratch copies of the fields before the warm-up
     ap.add_argument("--settle", type=int, default=200)
     ap.add_argument("--kernel-reps", type=int, default=200,
                     help="back-to-back launches of each step kernel for the roofline")
@@ -755,7 +763,7 @@ def main():
                          "workload")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
-                    help="only launch the one-, two- and three-step kernels (FP64, FP32) at --size "
+                    help="only launch the one- to four-step kernels (FP64, FP32) at --size "
                          "a few times each, for rocprofv3 counter runs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain device copy of 4 x 4098^2 FP64 arrays (1.07 GB moved per "
@@ -804,8 +812,8 @@ def main():
         return
 
     if args.kernels_only:
-        # one-, two- and three-step launches at --size, FP64 then FP32, for
-        # counter collection (rocprofv3 --pmc); prints one JSON line
+        # one-, two-, three- and four-step launches at --size, FP64 then
+        # FP32, for counter collection (rocprofv3 --pmc); prints one JSON line
         import torch
 
         from pfpa import model as M
@@ -815,13 +823,13 @@ def main():
         f0 = M.init_enriched_band(N, N, 1, seed=42)
         for dt in (torch.float64, torch.float32):
             sim = Simulation(s, f0, dtype=dt)
-            for k, n in ((1, 1), (2, 20), (3, 21)):
+            for k, n in ((1, 1), (2, 20), (3, 21), (4, 20)):
                 sim.steps_per_launch = k
                 sim.step(n)
             torch.cuda.synchronize()
             sim.close()
-        print(json.dumps({"kernels_only": [N, N], "launches": {"one_step": 1, "two_step": 10,
-                                                               "three_step": 7}}))
+        print(json.dumps({"kernels_only": [N, N], "launches": {
+            "one_step": 1, "two_step": 10, "three_step": 7, "four_step": 5}}))
         return
 
     if args.form == "precision":
@@ -838,8 +846,8 @@ def main():
     args.size = args.size or 4096
     t_all = time.perf_counter()
     plain = plain_section(args)
-    log("plain: %.4g cell-updates/s, %.1f us/launch three-step kernel" %
-        (plain["value"], plain["roofline"]["kernel_avg_us"]))
+    log("plain: %.4g cell-updates/s, %.1f us/launch %s" %
+        (plain["value"], plain["roofline"]["kernel_avg_us"], plain["roofline"]["kernel"]))
     out = {
         "metric": METRIC,
         "value": plain["value"],

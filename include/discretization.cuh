@@ -60,14 +60,19 @@ void device_set_boundary(struct CudaData* dev, int bc);
 /** Seed of the nucleation stream (default 42); resets the stream position. */
 void device_seed_prng(struct CudaData* dev, uint64_t seed);
 
-/** Steps per launch of device_evolution: 3 (default) or 1.  With 3, calls to
- * device_evolution are grouped: three consecutive steps (the driver swapping
- * the old/new pointers in between, src/main.cpp:318-321) run as one launch of
- * the three-step kernel, reading the first step's old fields and writing the
- * third step's new fields.  Every function of this API that reads the fields
- * (device_nucleation, device_dataviz, read_out_result, free_cuda) first runs
- * the pending steps one at a time, so what the driver observes is unchanged.
- * Code that reads the device arrays by other means calls device_flush first. */
+/** Steps per launch of device_evolution: 4 (default), 3 or 1.  With 4 or 3,
+ * calls to device_evolution are grouped: consecutive steps (the driver
+ * swapping the old/new pointers in between, src/main.cpp:318-321) run as
+ * multi-step launches, each reading one pointer set and writing the other.
+ * With 4, eight steps run as two four-step launches (an even number of
+ * launches, as eight swaps leave the newest fields in the first step's old
+ * set); with 3, three steps as one three-step launch.  Every function of this
+ * API that reads the fields (device_nucleation, device_dataviz,
+ * read_out_result, free_cuda) first runs the pending steps, as launches whose
+ * number has the parity of the steps (pairs of four-step launches, three-step
+ * launches, one-step launches), so what the driver observes is unchanged.
+ * Grids below 4x4 interior cells group three steps per launch.  Code that
+ * reads the device arrays by other means calls device_flush first. */
 void device_set_steps_per_launch(struct CudaData* dev, int steps);
 
 /** Queue the steps device_evolution has deferred (see above).  The steps run

@@ -130,6 +130,19 @@ int pf_step3_f32(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm,
                  const float* phi_del_old, const float* phi_lav_old,
                  float* conc_Cr_new, float* conc_Nb_new,
                  float* phi_del_new, float* phi_lav_new, void* stream);
+/* Four fused steps, old -> new: the same as four pf_step_* calls without
+ * nucleation (src/main.cpp:289-305, 318-321 run four times).  Same
+ * restrictions as pf_step2_*, and the interior must be at least 4x4. */
+int pf_step4_f64(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm,
+                 const double* conc_Cr_old, const double* conc_Nb_old,
+                 const double* phi_del_old, const double* phi_lav_old,
+                 double* conc_Cr_new, double* conc_Nb_new,
+                 double* phi_del_new, double* phi_lav_new, void* stream);
+int pf_step4_f32(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm,
+                 const float* conc_Cr_old, const float* conc_Nb_old,
+                 const float* phi_del_old, const float* phi_lav_old,
+                 float* conc_Cr_new, float* conc_Nb_new,
+                 float* phi_del_new, float* phi_lav_new, void* stream);
 
 /* Standalone nucleation pass in place on the given fields.  Replaces
  * device_nucleation (src/discretization.cu:1179-1209). */
@@ -208,10 +221,11 @@ typedef struct pf_step_item {
 	double* new_fields[4];
 } pf_step_item;
 int pf_step_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream);
-/* The same loop with pf_step2_f64 / pf_step3_f64 (two / three steps per item
+/* The same loop with pf_step2_f64 / pf_step3_f64 / pf_step4_f64 (two / three / four steps per item
  * and launch). */
 int pf_step2_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream);
 int pf_step3_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream);
+int pf_step4_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream);
 
 /* Variable-mobility Cahn-Hilliard divergence: the face-averaged
  * discrete_laplacian sums of chemical_convolution_Cr_kernel /
@@ -242,13 +256,15 @@ int pf_mobility_divergence_f64(int nx, int ny, double dx, double dy, const pf_mo
 
 /* Heuristic rows-per-block the step kernel uses for an nx x ny grid. */
 int pf_default_rows(int nx, int ny);
-/* Tile of the two-step (three-step) kernel: rows per chunk (0 = heuristic)
- * and prefetch depth (2, 3, 6; 0 = 3).  pf_default_rows2 (3) is its heuristic
- * for storage PF_STORAGE_F64 or PF_STORAGE_F32. */
+/* Tile of the two-step (three-, four-step) kernel: rows per chunk (0 =
+ * heuristic) and prefetch depth (2, 3, 6; 0 = 3).  pf_default_rows2 (3, 4) is
+ * its heuristic for storage PF_STORAGE_F64 or PF_STORAGE_F32. */
 int pf_ctx_set_tile2(pf_ctx* ctx, int rows, int depth);
 int pf_default_rows2(int nx, int ny, int storage);
 int pf_ctx_set_tile3(pf_ctx* ctx, int rows, int depth);
 int pf_default_rows3(int nx, int ny, int storage);
+int pf_ctx_set_tile4(pf_ctx* ctx, int rows, int depth);
+int pf_default_rows4(int nx, int ny, int storage);
 
 /* ------------------------------------------------------------------------
  * Batched form: many independent simulations of uneven size advanced by one

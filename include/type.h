@@ -84,8 +84,8 @@ struct CudaData {
 	fp_t kappa[NP], omega[NP], Lmob[NP];
 	fp_t mask[9];
 	/* grouped launches: steps issued by device_evolution and not yet run */
-	int group;           /* steps per launch (1 or 3) */
-	int pend;            /* pending steps (0..2) */
+	int group;           /* steps per launch (1, 3 or 4) */
+	int pend;            /* pending steps (0..7) */
 	fp_t* pend_old[4];   /* old and new fields of the first pending step */
 	fp_t* pend_new[4];
 	fp_t pend_alpha, pend_dt;

@@ -118,6 +118,8 @@ int stepk_t(int k, pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, co
   if (int r = check_grid(nx, ny)) return r;
   if (nx < 5 || ny < 5)
     return fail(PF_ERR_ARG, "multi-step kernels need at least 3x3 interior cells");
+  if (k == 4 && (nx < 6 || ny < 6))
+    return fail(PF_ERR_ARG, "the four-step kernel needs at least 4x4 interior cells");
   if (int r = check_ptrs(iCr, iNb, ipd, ipl)) return r;
   if (int r = check_ptrs(oCr, oNb, opd, opl)) return r;
   if (bc != PF_BC_NOFLUX && bc != PF_BC_PERIODIC) return fail(PF_ERR_ARG, "unknown boundary mode");
@@ -130,7 +132,8 @@ int stepk_t(int k, pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, co
                                          prm->Lmob, prm->alpha, prm->mask, storage_offset<T>(ctx));
   hipError_t e = pf::launch_stepk<T>(k, nx, ny, bc, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl,
                                      ctx->tile, (hipStream_t)stream);
-  if (e != hipSuccess) return hip_fail(e, k == 3 ? "three-step kernel" : "two-step kernel");
+  if (e != hipSuccess)
+    return hip_fail(e, k == 4 ? "four-step kernel" : k == 3 ? "three-step kernel" : "two-step kernel");
   return PF_OK;
 }
 
@@ -139,8 +142,8 @@ int set_tile_k(pf_ctx* ctx, int k, int rows, int depth) {
   if (rows < 0 || rows > 1 << 20) return fail(PF_ERR_ARG, "rows out of range");
   if (depth != 0 && depth != 2 && depth != 3 && depth != 6)
     return fail(PF_ERR_ARG, "prefetch depth must be 2, 3 or 6 (0 = default)");
-  (k == 3 ? ctx->tile.rows3 : ctx->tile.rows2) = rows;
-  (k == 3 ? ctx->tile.depth3 : ctx->tile.depth2) = depth ? depth : 3;
+  (k == 4 ? ctx->tile.rows4 : k == 3 ? ctx->tile.rows3 : ctx->tile.rows2) = rows;
+  (k == 4 ? ctx->tile.depth4 : k == 3 ? ctx->tile.depth3 : ctx->tile.depth2) = depth ? depth : 3;
   return PF_OK;
 }
 
@@ -274,12 +277,16 @@ int pf_step_f32(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const
 
 int pf_ctx_set_tile2(pf_ctx* ctx, int rows, int depth) { return set_tile_k(ctx, 2, rows, depth); }
 int pf_ctx_set_tile3(pf_ctx* ctx, int rows, int depth) { return set_tile_k(ctx, 3, rows, depth); }
+int pf_ctx_set_tile4(pf_ctx* ctx, int rows, int depth) { return set_tile_k(ctx, 4, rows, depth); }
 
 int pf_default_rows2(int nx, int ny, int storage) {
   return pf::default_rows_k(2, nx, ny, storage == PF_STORAGE_F32 ? 4 : 8);
 }
 int pf_default_rows3(int nx, int ny, int storage) {
   return pf::default_rows_k(3, nx, ny, storage == PF_STORAGE_F32 ? 4 : 8);
+}
+int pf_default_rows4(int nx, int ny, int storage) {
+  return pf::default_rows_k(4, nx, ny, storage == PF_STORAGE_F32 ? 4 : 8);
 }
 
 int pf_step2_f64(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const double* a,
@@ -304,6 +311,18 @@ int pf_step3_f32(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, cons
                  const float* b, const float* c, const float* d, float* oa, float* ob,
                  float* oc, float* od, void* stream) {
   return stepk_t<float>(3, ctx, nx, ny, bc, prm, a, b, c, d, oa, ob, oc, od, stream);
+}
+
+int pf_step4_f64(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const double* a,
+                 const double* b, const double* c, const double* d, double* oa, double* ob,
+                 double* oc, double* od, void* stream) {
+  return stepk_t<double>(4, ctx, nx, ny, bc, prm, a, b, c, d, oa, ob, oc, od, stream);
+}
+
+int pf_step4_f32(pf_ctx* ctx, int nx, int ny, int bc, const pf_params* prm, const float* a,
+                 const float* b, const float* c, const float* d, float* oa, float* ob,
+                 float* oc, float* od, void* stream) {
+  return stepk_t<float>(4, ctx, nx, ny, bc, prm, a, b, c, d, oa, ob, oc, od, stream);
 }
 
 int pf_step_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream) {
@@ -338,6 +357,10 @@ int pf_step2_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* strea
 
 int pf_step3_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream) {
   return stepk_loop(3, ctx, n, items, stream);
+}
+
+int pf_step4_loop_f64(pf_ctx* ctx, int n, const pf_step_item* items, void* stream) {
+  return stepk_loop(4, ctx, n, items, stream);
 }
 
 int pf_nucleation_f64(pf_ctx* ctx, int nx, int ny, double* a, double* b, double* c, double* d,

@@ -139,7 +139,7 @@ void init_cuda(struct HostData* host, const int nx, const int ny, const int nm,
   for (fp_t** f : fields) (void)checkCuda(hipMalloc((void**)f, bytes));
   check_pf(pf_ctx_create(&dev->ctx), "init_cuda");
   dev->bc = PF_BC_NOFLUX;
-  dev->group = 3;
+  dev->group = 4;
   dev->pend = 0;
   for (int i = 0; i < NC; i++) {
     dev->DCr[i] = D_Cr[i];
@@ -224,27 +224,53 @@ static void one_step(struct CudaData* dev, int nx, int ny, fp_t alpha, fp_t dt,
            "device_evolution");
 }
 
-void device_flush(struct CudaData* dev) {
-  // pending steps alternate between the two sets: first old -> new, then back
-  for (int i = 0; i < dev->pend; i++) {
-    if (i % 2 == 0)
-      one_step(dev, dev->pend_nx, dev->pend_ny, dev->pend_alpha, dev->pend_dt, dev->pend_old,
-               dev->pend_new);
+// Runs the pending steps.  The driver swaps its pointers after every step, so
+// after p steps the newest fields are in the new set of the first step if p
+// is odd and in its old set if p is even.  A launch of k steps reads one set
+// and writes the other, so the launches alternate between the sets and their
+// number must have the parity of p: pairs of four-step launches (where the
+// grid and the steps-per-launch setting allow), then three-step launches,
+// then one-step launches.
+static void run_pending(struct CudaData* dev) {
+  const int nx = dev->pend_nx, ny = dev->pend_ny;
+  const pf_params P = params_of(dev, dev->pend_alpha, dev->pend_dt);
+  int left = dev->pend, launches = 0;
+  auto launch = [&](int k) {
+    fp_t* const* src = launches % 2 ? dev->pend_new : dev->pend_old;
+    fp_t* const* dst = launches % 2 ? dev->pend_old : dev->pend_new;
+    if (k == 1)
+      one_step(dev, nx, ny, dev->pend_alpha, dev->pend_dt, src, dst);
     else
-      one_step(dev, dev->pend_nx, dev->pend_ny, dev->pend_alpha, dev->pend_dt, dev->pend_new,
-               dev->pend_old);
+      check_pf((k == 4 ? pf_step4_f64 : pf_step3_f64)(dev->ctx, nx, ny, dev->bc, &P, src[0], src[1],
+                                                      src[2], src[3], dst[0], dst[1], dst[2],
+                                                      dst[3], dev->str_A),
+               "device_evolution");
+    launches++;
+    left -= k;
+  };
+  const bool four = dev->group == 4 && nx >= 6 && ny >= 6;
+  while (four && left >= 8) {
+    launch(4);
+    launch(4);
   }
+  while (left >= 3) launch(3);
+  while (left > 0) launch(1);
   dev->pend = 0;
+}
+
+void device_flush(struct CudaData* dev) {
+  if (dev->pend > 0) run_pending(dev);
 }
 
 void device_set_steps_per_launch(struct CudaData* dev, int steps) {
   device_flush(dev);
-  dev->group = steps == 3 ? 3 : 1;
+  dev->group = steps == 4 ? 4 : steps == 3 ? 3 : 1;
 }
 
 // One step, old -> new (src/discretization.cu:1000-1020 and the kernels the
-// driver calls before it).  Grouped: the call is recorded and three recorded
-// steps run as one three-step launch.
+// driver calls before it).  Grouped: the call is recorded, and eight recorded
+// steps run as two four-step launches (three as one three-step launch with
+// three steps per launch).
 void device_evolution(struct CudaData* dev, const int nx, const int ny, const int nm,
                       const int bx, const int by, const fp_t alpha, const fp_t dt) {
   (void)nm;
@@ -255,7 +281,7 @@ void device_evolution(struct CudaData* dev, const int nx, const int ny, const in
   fp_t* const cur_new[4] = {dev->conc_Cr_new, dev->conc_Nb_new, dev->phi_del_new,
                             dev->phi_lav_new};
   const bool five_point = pf::symmetric_five_point(dev->mask);
-  if (dev->group != 3 || !five_point || nx < 5 || ny < 5) {
+  if (dev->group == 1 || !five_point || nx < 5 || ny < 5) {
     device_flush(dev);
     one_step(dev, nx, ny, alpha, dt, cur_old, cur_new);
     return;
@@ -279,14 +305,7 @@ void device_evolution(struct CudaData* dev, const int nx, const int ny, const in
     dev->pend_nx = nx;
     dev->pend_ny = ny;
   }
-  if (++dev->pend == 3) {
-    const pf_params P = params_of(dev, alpha, dt);
-    check_pf(pf_step3_f64(dev->ctx, nx, ny, dev->bc, &P, dev->pend_old[0], dev->pend_old[1],
-                          dev->pend_old[2], dev->pend_old[3], dev->pend_new[0], dev->pend_new[1],
-                          dev->pend_new[2], dev->pend_new[3], dev->str_A),
-             "device_evolution");
-    dev->pend = 0;
-  }
+  if (++dev->pend == (dev->group == 4 && nx >= 6 && ny >= 6 ? 8 : 3)) run_pending(dev);
 }
 
 void device_init_prng(struct CudaData* dev, const int, const int, const int, const int,

@@ -106,6 +106,8 @@ struct TileConfig {
   int depth2 = 3;
   int rows3 = 0;  // and for the three-step kernel
   int depth3 = 3;
+  int rows4 = 0;  // and the four-step kernel
+  int depth4 = 3;
 };
 
 hipError_t scratch_alloc(Scratch& s);

@@ -35,6 +35,9 @@ constexpr int STRIP3 = WAVE - 6;  // and of the three-step march (the batched fo
 constexpr int WPB = 4;           // waves (strips) per block: adjacent strips of one row chunk
 constexpr int UNROLL = 6;        // multiple of the 3-row window and of every ring depth
 
+// a pointer into LDS (address space 3), for the multi-step march's row windows
+typedef double __attribute__((address_space(3)))* LdsPtr;
+
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
   // Hardware dispatches block b to XCD b % 8.  Give XCD x the contiguous
   // logical range [x*per + min(x, rem), ... + count_x).
@@ -450,7 +453,7 @@ __device__ __forceinline__ void march(
 
 
 // ---------------------------------------------------------------------------
-// K explicit steps per pass (temporal blocking, K = 2 or 3).  The reference
+// K explicit steps per pass (temporal blocking, K = 2, 3 or 4).  The reference
 // advances one step per sequence of launches and so reads and writes every
 // field once per step; here a wave marches its strip through K time levels at
 // once, so the old fields are read once and the fields K steps later written
@@ -469,6 +472,14 @@ __device__ __forceinline__ void march(
 // keeps a three-row window in registers, row computed at iteration t in slot
 // t % 3; the first 2K iterations (the prologue) are unrolled at compile time.
 //
+// With a window array `lw` in LDS (LW = true, the four-step kernel), each
+// level's row also goes to LDS at the end of the iteration that computed it
+// and comes back from there two iterations later as the row behind: the
+// register windows then hold two live rows per level, not three, which is
+// what lets four levels fit in the registers of two waves per SIMD.  Each
+// lane reads back only what it wrote itself, so no barrier is needed.  The
+// array holds 2 slots (iteration parity) x K levels x 4 values x 64 lanes.
+//
 // The intermediate levels are exactly what one-step launches would have
 // stored: values are rounded to the storage type before the next level uses
 // them, and values at positions outside the interior are the images the
@@ -486,13 +497,14 @@ __device__ __forceinline__ void march(
 // So K steps here give the same result as K one-step launches.  Symmetric
 // five-point masks, no nucleation (the caller runs nucleation steps one at a
 // time).
-template <typename T, int BC, int K, int DEPTH, int SW>
+template <typename T, int BC, int K, int DEPTH, int SW, bool LW>
 __device__ __forceinline__ void marchk_impl(
     const T* __restrict__ iCr, const T* __restrict__ iNb, const T* __restrict__ ipd,
     const T* __restrict__ ipl, T* __restrict__ oCr, T* __restrict__ oNb, T* __restrict__ opd,
     T* __restrict__ opl, int Nx, int Ny, int pitch, int strip, int chunk, int rows,
-    const StepParams& P) {
-  static_assert(K >= 2 && K <= 3, "two or three steps per pass");
+    const StepParams& P, LdsPtr lw) {
+  static_assert(K >= 2 && K <= 4, "two to four steps per pass");
+  static_assert(UNROLL % 2 == 0, "the LDS slot is the iteration parity");
   static_assert(UNROLL % DEPTH == 0, "ring depth must divide the unroll factor");
   // SW output columns per strip (64 - 2K, or fewer where the strips are laid
   // out for a deeper march); lanes K .. K+SW-1 store
@@ -522,6 +534,25 @@ __device__ __forceinline__ void marchk_impl(
   Centre cen[K][3];
   Raw<T> ring[DEPTH];
 
+  // LDS copy of level l's row of an iteration of parity `slot`
+  auto put = [&](int l, int slot, const Point& v) {
+    LdsPtr q = lw + ((l * 2 + slot) * 4) * WAVE + lane;
+    q[0] = v.pd;
+    q[WAVE] = v.pl;
+    q[2 * WAVE] = v.qCr;
+    q[3 * WAVE] = v.qNb;
+  };
+  // level l's row behind at iteration t (slot sb of the window; in LDS, the
+  // row stored at iteration t - 2, of the same parity)
+  auto behind = [&](int l, int sb, int par) -> Point {
+    if constexpr (LW) {
+      const LdsPtr q = lw + ((l * 2 + par) * 4) * WAVE + lane;
+      return Point{q[0], q[WAVE], q[2 * WAVE], q[3 * WAVE]};
+    } else {
+      return win[l][sb];
+    }
+  };
+
   // One step at the centre of level `lv`'s window (slots behind / centre /
   // ahead): the next level's values, rounded to storage, derived into `v`, `c`.
   auto level_up = [&](const Point& Sth, const Point& C, const Centre& cc, const Point& Nth,
@@ -541,12 +572,22 @@ __device__ __forceinline__ void marchk_impl(
 
   // Levels 1..K-1 at iteration t (slots are compile-time in both callers):
   // level l computes row y0-K+t-l.
-  auto middle_levels = [&](int sa, int sc, int sb, int lmax_active) {
+  auto middle_levels = [&](int sa, int sc, int sb, int lmax_active, int par) {
 #pragma unroll
     for (int l = 1; l < K; l++) {
       if (l > lmax_active) break;
-      level_up(win[l - 1][sb], win[l - 1][sc], cen[l - 1][sc], win[l - 1][sa], win[l][sa],
-               cen[l][sa]);
+      level_up(behind(l - 1, sb, par), win[l - 1][sc], cen[l - 1][sc], win[l - 1][sa],
+               win[l][sa], cen[l][sa]);
+    }
+  };
+  // end of iteration t: the rows of the active levels to LDS
+  auto put_rows = [&](int sa, int lmax_active, int par) {
+    if constexpr (LW) {
+#pragma unroll
+      for (int l = 0; l < K; l++) {
+        if (l > lmax_active) break;
+        put(l, par, win[l][sa]);
+      }
     }
   };
 
@@ -564,11 +605,13 @@ __device__ __forceinline__ void marchk_impl(
       const int sa = t % 3, sc = (t + 2) % 3, sb = (t + 1) % 3;
       derive<T>(P, pro[t], win[0][sa], cen[0][sa]);
       // level l is active from iteration 2l on
-      middle_levels(sa, sc, sb, t / 2);
+      middle_levels(sa, sc, sb, t / 2, t % 2);
+      put_rows(sa, t / 2, t % 2);
     }
   }
 
-  // ---- main loop: iteration t = 2K + i, output row y = y0 + i
+  // ---- main loop: iteration t = 2K + i, output row y = y0 + i (t and k
+  // have the same parity: 2K and UNROLL are even)
   for (int ib = 0;; ib += UNROLL) {
 #pragma unroll
     for (int k = 0; k < UNROLL; k++) {
@@ -578,9 +621,9 @@ __device__ __forceinline__ void marchk_impl(
       const int rs = k % DEPTH;
       derive<T>(P, ring[rs], win[0][sa], cen[0][sa]);  // old row y + K
       load(ring[rs], min(y + K + DEPTH, y_last));
-      middle_levels(sa, sc, sb, K - 1);
+      middle_levels(sa, sc, sb, K - 1, k % 2);
       // level K: row y from level K-1's rows y-1 (behind), y, y+1 (ahead)
-      const Point& Sth = win[K - 1][sb];
+      const Point Sth = behind(K - 1, sb, k % 2);
       const Point& C = win[K - 1][sc];
       const Point& Nth = win[K - 1][sa];
       const Point W = shift_left(C), E = shift_right(C);
@@ -592,18 +635,19 @@ __device__ __forceinline__ void marchk_impl(
         store_row(oCr, oNb, opd, opl, (size_t)(1 + y) * row_b, out_lane_b, nCr, nNb, npd, npl);
         store_ghost_images<T, BC>(oCr, oNb, opd, opl, gx, y, Nx, Ny, pitch, nCr, nNb, npd, npl);
       }
+      put_rows(sa, K - 1, k % 2);
     }
   }
 }
 
-template <typename T, int BC, int K, int DEPTH, int SW = WAVE - 2 * K>
+template <typename T, int BC, int K, int DEPTH, int SW = WAVE - 2 * K, bool LW = false>
 __device__ __forceinline__ void marchk(
     const T* __restrict__ iCr, const T* __restrict__ iNb, const T* __restrict__ ipd,
     const T* __restrict__ ipl, T* __restrict__ oCr, T* __restrict__ oNb, T* __restrict__ opd,
     T* __restrict__ opl, int Nx, int Ny, int pitch, int strip, int chunk, int rows,
-    const StepParams& P) {
-  marchk_impl<T, BC, K, DEPTH, SW>(iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny, pitch, strip,
-                                   chunk, rows, P);
+    const StepParams& P, LdsPtr lw = nullptr) {
+  marchk_impl<T, BC, K, DEPTH, SW, LW>(iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny, pitch,
+                                       strip, chunk, rows, P, lw);
 }
 
 }  // namespace pf

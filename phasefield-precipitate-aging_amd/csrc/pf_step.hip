@@ -101,7 +101,7 @@ hipError_t launch_step(int nx, int ny, int bc, const StepParams& P, const T* iCr
 #undef PF_LAUNCH
 }
 
-// ---- K steps per launch (marchk, K = 2 or 3) ----
+// ---- K steps per launch (marchk, K = 2, 3 or 4) ----
 
 template <typename T, int BC, int K, int DEPTH>
 __global__ __launch_bounds__(WAVE * WPB) void stepk_kernel(
@@ -118,18 +118,41 @@ __global__ __launch_bounds__(WAVE * WPB) void stepk_kernel(
                           b / strips, rows, P);
 }
 
+// Four steps: the rows behind of the four levels' windows live in LDS (32 KiB
+// per wave, 64 KiB per block of WPB = 4 waves), which caps the kernel at two
+// blocks, so two waves per SIMD, per CU; the register budget is set to match.
+template <typename T, int BC, int DEPTH>
+__global__ __launch_bounds__(WAVE * WPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void step4_kernel(const T* __restrict__ iCr, const T* __restrict__ iNb,
+                  const T* __restrict__ ipd, const T* __restrict__ ipl, T* __restrict__ oCr,
+                  T* __restrict__ oNb, T* __restrict__ opd, T* __restrict__ opl, int Nx, int Ny,
+                  int pitch, int strips, int chunks, int rows, int remap, StepParams P) {
+  constexpr int K = 4;
+  constexpr int PER_WAVE = K * 2 * 4 * WAVE;  // levels x parity x values x lanes
+  __shared__ double win_lds[WPB * PER_WAVE];
+  const int nwaves = strips * chunks;
+  const int nblocks = (nwaves + WPB - 1) / WPB;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));  // wave-uniform
+  const int b = (remap ? xcd_remap(blockIdx.x, nblocks) : (int)blockIdx.x) * WPB + w;
+  if (b >= nwaves) return;
+  marchk<T, BC, K, DEPTH, WAVE - 2 * K, true>(iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny,
+                                              pitch, b % strips, b / strips, rows, P,
+                                              (LdsPtr)win_lds + w * PER_WAVE);
+}
+
 int default_rows_k(int k, int nx, int ny, int storage_bytes) {
   // The intermediate levels recompute (K-1)K rows per chunk in all, so chunks
   // are longer than the one-step kernel's while the grid still fills the chip.
   // Two steps: FP64 storage is near the HBM bound and prefers more waves in
   // flight (16 rows at 4096^2); FP32 storage is bound by arithmetic and
   // prefers less recomputation (32).  Three steps are bound by arithmetic in
-  // both and take 48 rows at 4096^2.
+  // both and take 48 rows at 4096^2.  Four steps recompute more rows per
+  // chunk still and run at two waves per SIMD: 60 rows at 4096^2.
   const int Nx = nx - 2, Ny = ny - 2;
   const int sw = WAVE - 2 * k;
   const int strips = (Nx + sw - 1) / sw;
-  int rows = k == 3 ? 48 : (storage_bytes == 8 ? 16 : 32);
-  const long min_waves = k == 3 ? 4096 : 8192;
+  int rows = k == 4 ? 60 : k == 3 ? 48 : (storage_bytes == 8 ? 16 : 32);
+  const long min_waves = k >= 3 ? 4096 : 8192;
   while (rows > 6 && (long)strips * ((Ny + rows - 1) / rows) < min_waves) rows >>= 1;
   return rows;
 }
@@ -144,9 +167,14 @@ static hipError_t launch_stepk_t(int nx, int ny, const StepParams& P, const T* i
   const int strips = (Nx + sw - 1) / sw;
   const int chunks = (Ny + rows - 1) / rows;
   const int blocks = (strips * chunks + WPB - 1) / WPB;
-  hipLaunchKernelGGL((stepk_kernel<T, BC, K, DEPTH>), dim3(blocks), dim3(WAVE * WPB), 0, st,
-                     iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny, nx, strips, chunks, rows,
-                     remap, P);
+  if constexpr (K == 4)
+    hipLaunchKernelGGL((step4_kernel<T, BC, DEPTH>), dim3(blocks), dim3(WAVE * WPB), 0, st, iCr,
+                       iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny, nx, strips, chunks, rows, remap,
+                       P);
+  else
+    hipLaunchKernelGGL((stepk_kernel<T, BC, K, DEPTH>), dim3(blocks), dim3(WAVE * WPB), 0, st,
+                       iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, Nx, Ny, nx, strips, chunks, rows,
+                       remap, P);
   return hipGetLastError();
 }
 
@@ -154,8 +182,8 @@ template <typename T>
 hipError_t launch_stepk(int k, int nx, int ny, int bc, const StepParams& P, const T* iCr,
                         const T* iNb, const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd,
                         T* opl, const TileConfig& tc, hipStream_t st) {
-  const int rows = k == 3 ? tc.rows3 : tc.rows2;
-  const int depth = k == 3 ? tc.depth3 : tc.depth2;
+  const int rows = k == 4 ? tc.rows4 : k == 3 ? tc.rows3 : tc.rows2;
+  const int depth = k == 4 ? tc.depth4 : k == 3 ? tc.depth3 : tc.depth2;
 #define PF_LAUNCHK(BCV, KV, D)                                                             \
   return launch_stepk_t<T, BCV, KV, D>(nx, ny, P, iCr, iNb, ipd, ipl, oCr, oNb, opd, opl, \
                                        rows, tc.remap, st)
@@ -165,6 +193,10 @@ hipError_t launch_stepk(int k, int nx, int ny, int bc, const StepParams& P, cons
     if (depth == 6) PF_LAUNCHK(BCV, KV, 6); \
     PF_LAUNCHK(BCV, KV, 3);            \
   } while (0)
+  if (k == 4) {
+    if (bc == 0) PF_LAUNCHK_D(0, 4);
+    PF_LAUNCHK_D(1, 4);
+  }
   if (k == 3) {
     if (bc == 0) PF_LAUNCHK_D(0, 3);
     PF_LAUNCHK_D(1, 3);

@@ -66,10 +66,14 @@ PROTOTYPES = {
     "pf_step2_f32": (_i, [_vp, _i, _i, _i, ctypes.POINTER(PfParams)] + [_vp] * 8 + [_vp]),
     "pf_step3_f64": (_i, [_vp, _i, _i, _i, ctypes.POINTER(PfParams)] + [_vp] * 8 + [_vp]),
     "pf_step3_f32": (_i, [_vp, _i, _i, _i, ctypes.POINTER(PfParams)] + [_vp] * 8 + [_vp]),
+    "pf_step4_f64": (_i, [_vp, _i, _i, _i, ctypes.POINTER(PfParams)] + [_vp] * 8 + [_vp]),
+    "pf_step4_f32": (_i, [_vp, _i, _i, _i, ctypes.POINTER(PfParams)] + [_vp] * 8 + [_vp]),
     "pf_ctx_set_tile2": (_i, [_vp, _i, _i]),
     "pf_default_rows2": (_i, [_i, _i, _i]),
     "pf_ctx_set_tile3": (_i, [_vp, _i, _i]),
     "pf_default_rows3": (_i, [_i, _i, _i]),
+    "pf_ctx_set_tile4": (_i, [_vp, _i, _i]),
+    "pf_default_rows4": (_i, [_i, _i, _i]),
     "pf_nucleation_f64": (_i, [_vp, _i, _i] + [_vp] * 4 + [ctypes.POINTER(PfNucParams), _vp]),
     "pf_nucleation_f32": (_i, [_vp, _i, _i] + [_vp] * 4 + [ctypes.POINTER(PfNucParams), _vp]),
     "pf_nucleation_status": (_i, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_i)]),
@@ -84,6 +88,7 @@ PROTOTYPES = {
     "pf_step_loop_f64": (_i, [_vp, _i, ctypes.POINTER(PfStepItem), _vp]),
     "pf_step2_loop_f64": (_i, [_vp, _i, ctypes.POINTER(PfStepItem), _vp]),
     "pf_step3_loop_f64": (_i, [_vp, _i, ctypes.POINTER(PfStepItem), _vp]),
+    "pf_step4_loop_f64": (_i, [_vp, _i, ctypes.POINTER(PfStepItem), _vp]),
     "pf_mobility_divergence_f64": (_i, [_i, _i, _d, _d, ctypes.POINTER(PfMobilityFields), _vp,
                                         _vp, _vp]),
     "pf_host_p": (_d, [_d]),

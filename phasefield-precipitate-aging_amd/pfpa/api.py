@@ -20,12 +20,14 @@ Differences from the reference, by design:
   * device_evolution runs one fused kernel that performs the whole step
     (boundaries, fictitious compositions, Laplacians, Cahn-Hilliard and
     Allen-Cahn); the five calls before it only record that they were made.
-  * Calls to device_evolution are grouped three at a time (as in the C++
-    drop-in): three consecutive steps, the caller swapping the old/new
-    fields in between, run as one launch of the three-step kernel.  Every
-    function here that reads the fields first runs the pending steps one at a
-    time, so what the caller observes is unchanged; device_flush runs them
-    explicitly and device_set_steps_per_launch(dev, 1) turns grouping off.
+  * Calls to device_evolution are grouped (as in the C++ drop-in): eight
+    consecutive steps, the caller swapping the old/new fields in between, run
+    as two launches of the four-step kernel (an even number of launches, so
+    the result lands in the set the caller's swaps put it in).  Every
+    function here that reads the fields first runs the pending steps, so what
+    the caller observes is unchanged; device_flush runs them explicitly,
+    device_set_steps_per_launch(dev, 3) groups three steps per three-step
+    launch instead and device_set_steps_per_launch(dev, 1) turns grouping off.
   * The per-cell cuRAND state (seeded from clock()) is replaced by a
     counter-based Philox4x32-10 stream: device_init_prng(seed=...) sets the
     key, and each device_nucleation call advances the stream position.
@@ -83,7 +85,7 @@ class CudaData:
         self.seed = DEFAULT_SEED
         self.nuc_calls = 0
         self.pending = []
-        self.group = 3        # steps per launch of grouped device_evolution calls
+        self.group = 4        # steps per launch of grouped device_evolution calls
         self.queued = 0       # steps recorded and not yet run (0..2)
         self.first = None     # (old, new, alpha, dt, nx, ny) of the first of them
 
@@ -232,31 +234,57 @@ def _launch(dev, fn, nx, ny, alpha, dt, old, new, *extra):
              *extra, _stream()), "device_evolution")
 
 
+def pending_launches(p, four):
+    """Steps per launch of the launches that run p pending steps.  After p
+    steps the newest fields are in the first step's new set if p is odd and
+    in its old set if p is even; launches alternate between the two sets, so
+    their number has the parity of p: pairs of four-step launches (`four`),
+    then three-step launches, then one-step launches."""
+    out = []
+    while four and p >= 8:
+        out += [4, 4]
+        p -= 8
+    while p >= 3:
+        out.append(3)
+        p -= 3
+    return out + [1] * p
+
+
+def group_size(group, nx, ny):
+    """Pending steps that trigger the launches: eight (two four-step
+    launches) with four steps per launch on a grid of at least 4x4 interior
+    cells, else three."""
+    return 8 if group == 4 and nx >= 6 and ny >= 6 else 3
+
+
 def device_flush(dev):
-    """Run the steps device_evolution has recorded and not yet launched, one
-    at a time: first old -> new, then back."""
+    """Run the steps device_evolution has recorded and not yet launched
+    (pending_launches: first old -> new, then back)."""
     if dev.queued:
         old, new, alpha, dt, nx, ny = dev.first
-        for i in range(dev.queued):
+        fns = {1: lib().pf_step_f64, 3: lib().pf_step3_f64, 4: lib().pf_step4_f64}
+        four = group_size(dev.group, nx, ny) == 8
+        for i, k in enumerate(pending_launches(dev.queued, four)):
             a, b = (old, new) if i % 2 == 0 else (new, old)
-            _launch(dev, lib().pf_step_f64, nx, ny, alpha, dt, a, b, None)
+            _launch(dev, fns[k], nx, ny, alpha, dt, a, b, *([None] if k == 1 else []))
     dev.queued = 0
     dev.first = None
 
 
 def device_set_steps_per_launch(dev, steps):
-    """3 (default): group device_evolution calls three per launch; 1: one."""
+    """4 (default): group device_evolution calls eight to two four-step
+    launches; 3: three per three-step launch; 1: one per launch."""
     device_flush(dev)
-    dev.group = 3 if steps == 3 else 1
+    dev.group = steps if steps in (3, 4) else 1
 
 
 def device_evolution(dev, nx, ny, nm, bx, by, alpha, dt):
     """src/discretization.cu:982-1020: one fused step *_old -> *_new, grouped
-    three to a launch (see the module docstring)."""
+    with the calls around it (see the module docstring)."""
     dev.pending.clear()
     old, new = _fields(dev, "_old"), _fields(dev, "_new")
     five_point = M.symmetric_five_point(dev.params.mask)
-    if dev.group != 3 or not five_point or nx < 5 or ny < 5:
+    if dev.group == 1 or not five_point or nx < 5 or ny < 5:
         device_flush(dev)
         _launch(dev, lib().pf_step_f64, nx, ny, alpha, dt, old, new, None)
         return
@@ -270,11 +298,8 @@ def device_evolution(dev, nx, ny, nm, bx, by, alpha, dt):
     if not dev.queued:
         dev.first = (old, new, alpha, dt, nx, ny)
     dev.queued += 1
-    if dev.queued == 3:
-        f_old, f_new = dev.first[0], dev.first[1]
-        _launch(dev, lib().pf_step3_f64, nx, ny, alpha, dt, f_old, f_new)
-        dev.queued = 0
-        dev.first = None
+    if dev.queued == group_size(dev.group, nx, ny):
+        device_flush(dev)
 
 
 def nuc_params(dev, sigma_del, sigma_lav, unit_a, ifce_width, dx, dy, dz, dt):

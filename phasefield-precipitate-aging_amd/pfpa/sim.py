@@ -39,6 +39,23 @@ def iter_launches(t0, n, max_steps, nuc_every=0):
         t += k
 
 
+def default_steps_per_launch(setup, dtype, requested=None):
+    """Steps per launch of a Simulation: `requested` (1 to 4), or by default
+    four for FP64 storage and three for FP32 (the faster kernel of each on
+    the grids measured), reduced to what the grid and mask allow: the
+    multi-step kernels take symmetric five-point masks and an interior of at
+    least 3x3 (four steps: 4x4)."""
+    if requested is None:
+        requested = 4 if dtype == torch.float64 else 3
+    if requested not in (1, 2, 3, 4):
+        raise ValueError("steps_per_launch must be 1, 2, 3 or 4")
+    if not (M.symmetric_five_point(setup.mask) and setup.nx >= 5 and setup.ny >= 5):
+        return 1
+    if requested == 4 and (setup.nx < 6 or setup.ny < 6):
+        return 3
+    return requested
+
+
 def plan_launches(t0, n, max_steps, nuc_every=0):
     """iter_launches as a list."""
     return list(iter_launches(t0, n, max_steps, nuc_every))
@@ -127,15 +144,9 @@ class Simulation:
         self._step = lib().pf_step_f64 if dtype == torch.float64 else lib().pf_step_f32
         f64 = dtype == torch.float64
         self._stepk = {2: lib().pf_step2_f64 if f64 else lib().pf_step2_f32,
-                       3: lib().pf_step3_f64 if f64 else lib().pf_step3_f32}
-        # three steps per launch: 1/3 of the one-step HBM traffic per step
-        if steps_per_launch is None:
-            steps_per_launch = 3
-        if steps_per_launch not in (1, 2, 3):
-            raise ValueError("steps_per_launch must be 1, 2 or 3")
-        if not (M.symmetric_five_point(setup.mask) and self.nx >= 5 and self.ny >= 5):
-            steps_per_launch = 1
-        self.steps_per_launch = steps_per_launch
+                       3: lib().pf_step3_f64 if f64 else lib().pf_step3_f32,
+                       4: lib().pf_step4_f64 if f64 else lib().pf_step4_f32}
+        self.steps_per_launch = default_steps_per_launch(setup, dtype, steps_per_launch)
         self._summ = lib().pf_summarize_f64 if dtype == torch.float64 else lib().pf_summarize_f32
 
     def set_tile(self, rows, remap=1, depth=0):
@@ -147,16 +158,29 @@ class Simulation:
     def set_tile3(self, rows, depth=0):
         check(lib().pf_ctx_set_tile3(self.ctx, rows, depth), "pf_ctx_set_tile3")
 
+    def set_tile4(self, rows, depth=0):
+        check(lib().pf_ctx_set_tile4(self.ctx, rows, depth), "pf_ctx_set_tile4")
+
     def tune_tile3(self, rows_set=(6, 8, 12, 16, 24, 32, 48, 64, 72), launches=20):
-        """Time the three-step kernel at each chunk height in `rows_set` on
-        scratch copies of the current fields (the simulation does not advance)
-        and keep the fastest.  The default height is a heuristic within a few
-        per cent of the best on the grids measured; this measures the grid at
-        hand.  Results do not depend on the tile.  Returns (rows, {rows: us
-        per launch})."""
+        """tune_tile for the three-step kernel (the simulation must run three
+        steps per launch)."""
         if self.steps_per_launch != 3:
             raise ValueError("tune_tile3 needs three steps per launch")
-        fn = self._stepk[3]
+        return self.tune_tile(rows_set, launches)
+
+    def tune_tile(self, rows_set=(6, 8, 12, 16, 24, 32, 48, 60, 64, 72), launches=20):
+        """Time the simulation's multi-step kernel (steps_per_launch 2, 3 or
+        4) at each chunk height in `rows_set` on scratch copies of the
+        current fields (the simulation does not advance) and keep the
+        fastest.  The default height is a heuristic within a few per cent of
+        the best on the grids measured; this measures the grid at hand.
+        Results do not depend on the tile.  Returns (rows, {rows: us per
+        launch})."""
+        k = self.steps_per_launch
+        if k < 2:
+            raise ValueError("tune_tile needs two or more steps per launch")
+        fn = self._stepk[k]
+        set_tile = {2: self.set_tile2, 3: self.set_tile3, 4: self.set_tile4}[k]
         st = ctypes.c_void_p(self.stream.cuda_stream)
         # the copies are made on the simulation's stream, after the steps
         # queued there, and the kernels that read them run there too
@@ -168,11 +192,11 @@ class Simulation:
             for i in range(n):
                 x, y = (a, b) if i % 2 == 0 else (b, a)
                 check(fn(self.ctx, self.nx, self.ny, self.s.bc, ctypes.byref(self.P),
-                         *[_ptr(t) for t in x], *[_ptr(t) for t in y], st), "pf_step3")
+                         *[_ptr(t) for t in x], *[_ptr(t) for t in y], st), "pf_step%d" % k)
 
         times = {}
         for rows in rows_set:
-            self.set_tile3(rows, 3)
+            set_tile(rows, 3)
             run(2)
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
@@ -182,7 +206,7 @@ class Simulation:
             e1.synchronize()
             times[rows] = e0.elapsed_time(e1) * 1e3 / launches
         best = min(times, key=times.get)
-        self.set_tile3(best, 3)
+        set_tile(best, 3)
         return best, times
 
     def step(self, n=1):

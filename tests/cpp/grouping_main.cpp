@@ -1,11 +1,13 @@
 // grouping_main.cpp -- the drop-in's grouping of device_evolution calls
-// (three to a launch) under call sequences the reference's driver does not
+// (eight to two four-step launches, or three to a three-step launch) under
+// call sequences the reference's driver does not
 // make but a caller of the same API may: a change of dt inside a group, a
 // step whose pointers were not swapped after the previous one (the same
 // old -> new twice), a nucleation pass, and a direct read of the device arrays
-// after device_flush.  The scripted sequence runs twice, grouped and with one
-// launch per call (device_set_steps_per_launch(&dev, 1)); the final fields and
-// the directly read intermediate must be the same bits.
+// after device_flush, and pending tails of every length the sequence reaches.
+// The scripted sequence runs three times: four and three steps per launch,
+// and one launch per call (device_set_steps_per_launch(&dev, 1)); the final
+// fields and the directly read intermediate must be the same bits.
 //
 //   grouping_main <in.bin> <dt> <dx>
 //
@@ -98,7 +100,10 @@ static int run(const char* in, double dt, double meshres, int steps_per_launch, 
   if (hipMemcpy(out.mid.data(), dev.conc_Cr_old, sizeof(double) * nx * ny,
                 hipMemcpyDeviceToHost) != hipSuccess)
     return 5;
-  for (int s = 12; s < 20; s++) step(dt, true);  // 12..19: two groups and a tail
+  // 12..34: with four steps per launch two groups of eight and a tail of
+  // seven (three-, three- and one-step launches); with three, seven groups and
+  // a tail of two
+  for (int s = 12; s < 35; s++) step(dt, true);
   read_out_result(&dev, &host, nx, ny);
 
   out.fields.resize((size_t)4 * nx * ny);
@@ -116,17 +121,20 @@ int main(int argc, char* argv[]) {
     return 2;
   }
   const double dt = atof(argv[2]), dx = atof(argv[3]);
-  Result grouped, single;
-  if (int r = run(argv[1], dt, dx, 3, grouped)) return r;
+  Result single;
   if (int r = run(argv[1], dt, dx, 1, single)) return r;
-  if (memcmp(grouped.mid.data(), single.mid.data(), sizeof(double) * grouped.mid.size()) != 0) {
-    fprintf(stderr, "direct read after device_flush differs\n");
-    return 1;
-  }
-  if (memcmp(grouped.fields.data(), single.fields.data(),
-             sizeof(double) * grouped.fields.size()) != 0) {
-    fprintf(stderr, "final fields differ\n");
-    return 1;
+  for (int spl : {4, 3}) {
+    Result grouped;
+    if (int r = run(argv[1], dt, dx, spl, grouped)) return r;
+    if (memcmp(grouped.mid.data(), single.mid.data(), sizeof(double) * grouped.mid.size()) != 0) {
+      fprintf(stderr, "%d steps per launch: direct read after device_flush differs\n", spl);
+      return 1;
+    }
+    if (memcmp(grouped.fields.data(), single.fields.data(),
+               sizeof(double) * grouped.fields.size()) != 0) {
+      fprintf(stderr, "%d steps per launch: final fields differ\n", spl);
+      return 1;
+    }
   }
   printf("grouping ok\n");
   return 0;

@@ -56,8 +56,8 @@ def run_harness(f0, s, steps, nuc, io, spl=None, bc=None, mask_code=None):
 
 def test_cpp_driver_replay(gpu, oracle):
     """The reference's call sequence with nucleation every 100 steps and I/O
-    every 150: grouped (three steps per launch, the default) and one launch per
-    call give the same bits, and both match the oracle."""
+    every 150: grouped (four steps per launch, the default; and three) and one
+    launch per call give the same bits, and all match the oracle."""
     assert os.path.exists(HARNESS), "tests/cpp/replay_main not built: run __graft_entry__.build()"
     Nx, Ny = 120, 90
     s = M.make_setup(Nx, Ny, bc=0)
@@ -65,7 +65,9 @@ def test_cpp_driver_replay(gpu, oracle):
     steps = 300
     arr = run_harness(f0, s, steps, 100, 150)
     one = run_harness(f0, s, steps, 100, 150, spl=1)
+    three = run_harness(f0, s, steps, 100, 150, spl=3)
     want = oracle_reference(oracle, f0, s, steps)
+    assert np.array_equal(three, one)
     for k in range(4):
         assert np.array_equal(arr[k], one[k]), k
         assert mixed_err(arr[k], want[k]) <= TOL, k
@@ -90,22 +92,24 @@ def test_cpp_driver_replay_boundary_and_mask(gpu, oracle, bc, mask_code):
     steps = 40
     a = run_harness(f0, s, steps, 0, 0, spl=3, bc=bc, mask_code=mask_code)
     b = run_harness(f0, s, steps, 0, 0, spl=1, bc=bc, mask_code=mask_code)
+    c = run_harness(f0, s, steps, 0, 0, spl=4, bc=bc, mask_code=mask_code)
     P = oracle.make_params(s.dt, s.DCr, s.DNb, s.kappa, s.omega, s.Lmob, s.alpha, s.mask)
     want = oracle.run(f0, P, bc, steps)
     for k in range(4):
         assert np.array_equal(a[k], b[k]), k
+        assert np.array_equal(c[k], b[k]), k
         assert mixed_err(a[k], want[k]) <= TOL, k
 
 
 def test_cpp_driver_replay_4096(gpu, oracle):
     """The drop-in benchmark's grid: the reference's call sequence on 4096^2
     (no-flux, the enriched-band workload, nucleation every 891 steps, I/O at
-    the last step), 7 steps grouped three to a launch, every cell against the
-    oracle."""
+    the last step), 11 steps grouped eight to two four-step launches and a
+    three-step launch, every cell against the oracle."""
     assert os.path.exists(HARNESS), "tests/cpp/replay_main not built: run __graft_entry__.build()"
     s = M.make_setup(4096, 4096, bc=0)
     f0 = M.init_enriched_band(4096, 4096, 0, seed=42)
-    steps = 7
+    steps = 11
     arr = run_harness(f0, s, steps, 891, steps)
     want = oracle_reference(oracle, f0, s, steps)
     for k in range(4):
@@ -121,13 +125,15 @@ def test_cpp_driver_replay_frequent_io(gpu, io):
     f0 = M.init_matrix_with_particles(Nx, Ny, s.dx, 1, seed=9, radius=1.2e-9)
     a = run_harness(f0, s, 23, 5, io)      # 23 steps: a pending tail flushed by read-out
     b = run_harness(f0, s, 23, 5, io, spl=1)
+    c = run_harness(f0, s, 23, 5, io, spl=3)
     assert np.array_equal(a, b)
+    assert np.array_equal(c, b)
 
 
-@pytest.mark.parametrize("spl", [3, 1])
+@pytest.mark.parametrize("spl", [4, 3, 1])
 def test_python_api_replay(gpu, oracle, spl):
     """pfpa.api in the reference's call sequence, I/O steps included, with
-    grouped launches (3, the default) and one launch per call."""
+    grouped launches (4, the default, and 3) and one launch per call."""
     Nx, Ny = 100, 70
     s = M.make_setup(Nx, Ny, bc=0)
     f0 = M.init_matrix_with_particles(Nx, Ny, s.dx, 0, seed=8, radius=2e-9)
@@ -246,7 +252,7 @@ def _python_grouping_sequence(spl, s, f0):
     api.device_flush(dev)
     torch.cuda.synchronize()
     mid = dev.conc_Cr_old.cpu().numpy().copy()
-    for _ in range(8):
+    for _ in range(23):         # two groups of eight and a tail of seven at 4
         step(dt)
     api.read_out_result(dev, host, nx, ny)
     out = [host.conc_Cr_new.copy(), host.conc_Nb_new.copy(), host.phi_del_new.copy(),
@@ -258,8 +264,9 @@ def _python_grouping_sequence(spl, s, f0):
 def test_python_grouping_edge_cases(gpu):
     """The same sequence through the Python mirror (pfpa.api)."""
     s, f0 = _grouping_input(90, 64, 0, 12)
-    m3, o3 = _python_grouping_sequence(3, s, f0)
     m1, o1 = _python_grouping_sequence(1, s, f0)
-    assert np.array_equal(m3, m1)
-    for a, b in zip(o3, o1):
-        assert np.array_equal(a, b)
+    for spl in (4, 3):
+        m, o = _python_grouping_sequence(spl, s, f0)
+        assert np.array_equal(m, m1), spl
+        for a, b in zip(o, o1):
+            assert np.array_equal(a, b), spl

@@ -1,6 +1,7 @@
-"""The temporally blocked kernels, pf_step2_* and pf_step3_*, through the C-ABI.
+"""The temporally blocked kernels, pf_step2_*, pf_step3_* and pf_step4_*,
+through the C-ABI.
 
-Each must give what two (three) one-step launches give: the same interior
+Each must give what two (three, four) one-step launches give: the same interior
 values and the same ghost-ring images, for both boundary modes, both storage
 types, ragged grids (strips and chunks that do not divide the grid, chunks of
 one row, tiny grids where the images sit inside the first chunk) and every
@@ -62,16 +63,23 @@ def ctx(gpu):
     lib().pf_ctx_destroy(h)
 
 
-SHAPES = [(3, 3), (4, 5), (5, 4), (59, 7), (60, 61), (61, 62), (62, 3), (120, 9), (121, 33),
-          (200, 131), (37, 300)]
+# strip widths 64 - 2k: 60, 58 and 56 output columns
+SHAPES = [(3, 3), (4, 4), (4, 5), (5, 4), (56, 9), (57, 58), (59, 7), (60, 61), (61, 62),
+          (62, 3), (120, 9), (121, 33), (200, 131), (37, 300)]
+
+MULTI = {2: ("pf_step2_f64", "pf_step2_f32", "pf_ctx_set_tile2"),
+         3: ("pf_step3_f64", "pf_step3_f32", "pf_ctx_set_tile3"),
+         4: ("pf_step4_f64", "pf_step4_f32", "pf_ctx_set_tile4")}
 
 
-@pytest.mark.parametrize("k", [2, 3])
+@pytest.mark.parametrize("k", [2, 3, 4])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("bc", [0, 1])
 @pytest.mark.parametrize("shape", SHAPES)
 def test_k_steps_in_one_launch(ctx, shape, bc, dtype, k):
     Nx, Ny = shape
+    if k == 4 and min(Nx, Ny) < 4:
+        pytest.skip("the four-step kernel needs a 4x4 interior (rejected, see the error test)")
     nx, ny = Nx + 2, Ny + 2
     s = M.make_setup(Nx, Ny, bc=bc, dt_rule=4e-7)
     P = _params(s)
@@ -79,9 +87,8 @@ def test_k_steps_in_one_launch(ctx, shape, bc, dtype, k):
           _fields(Nx, Ny, bc, 1000 * Nx + Ny)]
     one = lib().pf_step_f64 if dtype == torch.float64 else lib().pf_step_f32
     f64 = dtype == torch.float64
-    multi = {2: lib().pf_step2_f64 if f64 else lib().pf_step2_f32,
-             3: lib().pf_step3_f64 if f64 else lib().pf_step3_f32}[k]
-    set_tile = {2: lib().pf_ctx_set_tile2, 3: lib().pf_ctx_set_tile3}[k]
+    multi = getattr(lib(), MULTI[k][0 if f64 else 1])
+    set_tile = getattr(lib(), MULTI[k][2])
     cur = f0
     for _ in range(k):
         nxt = [torch.full_like(x, np.nan) for x in f0]
@@ -109,7 +116,7 @@ def test_k_steps_in_one_launch(ctx, shape, bc, dtype, k):
 
 
 @pytest.mark.parametrize("bc", [0, 1])
-@pytest.mark.parametrize("spl", [2, 3])
+@pytest.mark.parametrize("spl", [2, 3, 4])
 def test_fused_run_matches_one_step_run(gpu, oracle, bc, spl):
     """A Simulation grouping steps through the multi-step kernels, with the
     reference's nucleation schedule (odd interval, so groups straddle the
@@ -133,12 +140,13 @@ def test_fused_run_matches_one_step_run(gpu, oracle, bc, spl):
     b.close()
 
 
-@pytest.mark.parametrize("k", [2, 3])
+@pytest.mark.parametrize("k", [2, 3, 4])
 def test_multistep_errors(ctx, k):
     s = M.make_setup(8, 8, bc=0)
     P = _params(s)
     f = [torch.zeros(10 * 10, dtype=torch.float64, device="cuda") for _ in range(8)]
-    two = {2: lib().pf_step2_f64, 3: lib().pf_step3_f64}[k]
+    two = getattr(lib(), MULTI[k][0])
+    set_tile = getattr(lib(), MULTI[k][2])
     # in place
     assert two(ctx, 10, 10, 0, ctypes.byref(P), *map(_p, f[:4]), *map(_p, f[:4]), None) != 0
     # nine-point mask
@@ -154,18 +162,24 @@ def test_multistep_errors(ctx, k):
     P5 = _params(s, m5)
     assert two(ctx, 10, 10, 0, ctypes.byref(P5), *map(_p, f[:4]), *map(_p, f[4:]), None) != 0
     assert b"symmetric five-point" in lib().pf_last_error()
-    # interior smaller than 3x3
+    # interior smaller than 3x3 (four steps: 4x4)
     assert two(ctx, 4, 10, 0, ctypes.byref(P), *map(_p, f[:4]), *map(_p, f[4:]), None) != 0
+    small = two(ctx, 5, 10, 0, ctypes.byref(P), *map(_p, f[:4]), *map(_p, f[4:]), None)
+    assert (small != 0) == (k == 4)
+    if k == 4:
+        assert b"4x4" in lib().pf_last_error()
     # bad boundary mode, bad tile
     assert two(ctx, 10, 10, 7, ctypes.byref(P), *map(_p, f[:4]), *map(_p, f[4:]), None) != 0
-    assert lib().pf_ctx_set_tile2(ctx, 8, 4) != 0
+    assert set_tile(ctx, 8, 4) != 0
+    assert set_tile(ctx, -1, 3) != 0
     assert two(ctx, 10, 10, 0, ctypes.byref(P), *map(_p, f[:4]), *map(_p, f[4:]), None) == 0
     torch.cuda.synchronize()
 
 
 def test_step_loops_equal_single_calls(gpu):
-    """pf_step_loop_f64 / pf_step2_loop_f64 / pf_step3_loop_f64 (one launch per
-    grid of a list, the batched form's comparison baseline) give the same bits
+    """pf_step_loop_f64 / pf_step2_loop_f64 / pf_step3_loop_f64 /
+    pf_step4_loop_f64 (one launch per grid of a list, the batched form's
+    comparison baseline) give the same bits
     as the single-grid entry points on each grid; a bad list is rejected."""
     from pfpa._lib import PfParams, PfStepItem, check, lib
     grids = [(40, 30, 0), (129, 17, 1), (7, 64, 1)]
@@ -196,7 +210,7 @@ def test_step_loops_equal_single_calls(gpu):
     ctx = ctypes.c_void_p()
     check(lib().pf_ctx_create(ctypes.byref(ctx)), "ctx")
     for name, single in (("pf_step_loop_f64", "pf_step_f64"), ("pf_step2_loop_f64", "pf_step2_f64"),
-                         ("pf_step3_loop_f64", "pf_step3_f64")):
+                         ("pf_step3_loop_f64", "pf_step3_f64"), ("pf_step4_loop_f64", "pf_step4_f64")):
         check(getattr(lib(), name)(ctx, len(grids), items, None), name)
         torch.cuda.synchronize()
         for (s, P, A), (_, _, B) in zip(singles, keep):
@@ -213,24 +227,30 @@ def test_step_loops_equal_single_calls(gpu):
     lib().pf_ctx_destroy(ctx)
 
 
-def test_tune_tile3_leaves_fields_and_results(gpu):
-    """Simulation.tune_tile3 measures on scratch copies: the fields do not move,
-    the chosen height is one of the candidates, and stepping afterwards gives
-    the same bits as an untuned simulation."""
+@pytest.mark.parametrize("spl", [3, 4])
+def test_tune_tile_leaves_fields_and_results(gpu, spl):
+    """Simulation.tune_tile (tune_tile3 at three steps per launch) measures on
+    scratch copies: the fields do not move, the chosen height is one of the
+    candidates, and stepping afterwards gives the same bits as an untuned
+    simulation."""
     s = M.make_setup(300, 200, bc=1)
     f0 = M.init_matrix_with_particles(300, 200, s.dx, 1, seed=4, radius=2e-9)
-    tuned = Simulation(s, f0)
+    tuned = Simulation(s, f0, steps_per_launch=spl)
     before = tuned.fields()
-    rows, times = tuned.tune_tile3(rows_set=(6, 16, 48), launches=4)
+    tune = tuned.tune_tile3 if spl == 3 else tuned.tune_tile
+    rows, times = tune(rows_set=(6, 16, 48), launches=4)
     assert rows in (6, 16, 48) and set(times) == {6, 16, 48}
     after = tuned.fields()
     for k in range(4):
         assert np.array_equal(before[k], after[k])
     tuned.step(31)
-    plain = Simulation(s, f0)
+    plain = Simulation(s, f0, steps_per_launch=spl)
     plain.step(31)
     for x, y in zip(tuned.fields(), plain.fields()):
         assert np.array_equal(x, y)
+    if spl == 4:
+        with pytest.raises(ValueError):
+            tuned.tune_tile3()
     tuned.close()
     plain.close()
 

@@ -608,7 +608,7 @@ def test_anisotropic_masks(gpu, oracle, bc, kind):
         assert abs(sum(ninem)) > 1e-3 * max(abs(v) for v in ninem)
     f0 = M.init_matrix_with_particles(Nx, Ny, dx, bc, seed=8, radius=2e-9)
     want = oracle.run(f0, oracle_params(oracle, s), bc, 25)
-    for spl in ((1,) if kind == "nine" else (1, 3)):
+    for spl in ((1,) if kind == "nine" else (1, 3, 4)):
         sim = Simulation(s, f0, steps_per_launch=spl)
         assert sim.steps_per_launch == (spl if kind == "five" else 1)
         sim.step(25)

@@ -11,7 +11,7 @@ from pfpa.sim import plan_launches
 from pfpa.workloads import batch_specs
 
 
-@pytest.mark.parametrize("max_steps", [1, 2, 3])
+@pytest.mark.parametrize("max_steps", [1, 2, 3, 4])
 @pytest.mark.parametrize("nuc_every", [0, 1, 2, 3, 4, 7, 891])
 @pytest.mark.parametrize("t0", [0, 1, 5, 890])
 def test_plan_launches(t0, max_steps, nuc_every):
@@ -37,14 +37,14 @@ def test_plan_launches(t0, max_steps, nuc_every):
 
 
 def _plan_brute_force(t0, n, max_steps, nuc_every):
-    """Per launch, the longest group of 3 or 2 steps with no nucleation step in
-    it, else one step (with the pass when it is a nucleation step)."""
+    """Per launch, the longest group of 4, 3 or 2 steps with no nucleation step
+    in it, else one step (with the pass when it is a nucleation step)."""
     def due(t):
         return nuc_every > 0 and t % nuc_every == 0
     out, t, left = [], t0, n
     while left > 0:
         k = 1
-        for cand in (3, 2):
+        for cand in (4, 3, 2):
             if cand <= max_steps and left >= cand and not any(due(t + j) for j in range(cand)):
                 k = cand
                 break
@@ -54,7 +54,7 @@ def _plan_brute_force(t0, n, max_steps, nuc_every):
     return out
 
 
-@pytest.mark.parametrize("max_steps", [1, 2, 3])
+@pytest.mark.parametrize("max_steps", [1, 2, 3, 4])
 def test_plan_launches_against_brute_force(max_steps):
     for nuc_every in (0, 1, 2, 3, 4, 5, 7, 891):
         for t0 in (0, 1, 2, 5, 889, 890, 891):
@@ -76,6 +76,55 @@ def test_plan_launches_schedule_of_the_benchmark():
         t += k
     assert nuc_steps == [0, 891]
     assert sum(1 for k, _ in plan if k == 3) >= 330
+
+
+def test_default_steps_per_launch():
+    """Four steps per launch for FP64 storage and three for FP32 by default;
+    an explicit request is kept where the grid and mask allow it, else cut to
+    three (interiors below 4x4) or one (below 3x3, or not a symmetric
+    five-point mask)."""
+    import torch
+
+    from pfpa.sim import default_steps_per_launch as spl
+
+    big, five, four = M.make_setup(64, 48, bc=1), M.make_setup(3, 9, bc=0), M.make_setup(4, 4, bc=1)
+    assert spl(big, torch.float64) == 4 and spl(big, torch.float32) == 3
+    for k in (1, 2, 3, 4):
+        assert spl(big, torch.float64, k) == k and spl(big, torch.float32, k) == k
+    assert spl(four, torch.float64) == 4
+    assert spl(five, torch.float64) == 3 and spl(five, torch.float64, 4) == 3
+    assert spl(five, torch.float64, 2) == 2
+    tiny = M.make_setup(2, 9, bc=0)
+    assert spl(tiny, torch.float64) == 1
+    nine = M.make_setup(64, 48, bc=1)
+    nine.mask = [1.0] * 4 + [-8.0] + [1.0] * 4
+    assert spl(nine, torch.float64) == 1
+    for bad in (0, 5, -1):
+        with pytest.raises(ValueError):
+            spl(big, torch.float64, bad)
+
+
+def test_dropin_pending_launches():
+    """The launches that run p pending drop-in steps: they add up to p, their
+    number has the parity of p (so the result lands in the pointer set the
+    caller's swaps put it in), four-step launches come in pairs and only when
+    allowed, and a full group is two four-step launches (eight steps) or one
+    three-step launch."""
+    from pfpa.api import group_size, pending_launches
+    for four in (False, True):
+        for p in range(0, 40):
+            ks = pending_launches(p, four)
+            assert sum(ks) == p
+            assert len(ks) % 2 == p % 2
+            assert all(k in ((1, 3, 4) if four else (1, 3)) for k in ks)
+            assert ks.count(4) % 2 == 0
+            assert ks == sorted(ks, reverse=True)
+            assert ks.count(1) <= 2
+    assert pending_launches(8, True) == [4, 4]
+    assert pending_launches(7, True) == [3, 3, 1]
+    assert pending_launches(3, False) == [3]
+    assert group_size(4, 6, 6) == 8 and group_size(4, 5, 100) == 3
+    assert group_size(3, 100, 100) == 3
 
 
 def test_batch_specs():
