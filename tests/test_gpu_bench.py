@@ -36,7 +36,9 @@ def test_bench_json_line():
     assert d["cpu_baseline"] is None
     # the timed loop and the kernel leg ran in the same clock state
     t = d["timing"]
-    assert t["four_step_kernel_us_per_step"] <= t["timed_loop_us_per_step"]
+    assert t["four_step_kernel_us_per_step"] <= 1.10 * t["timed_loop_us_per_step"]
+    assert t["kernel_leg_over_timed_loop"] == pytest.approx(
+        t["four_step_kernel_us_per_step"] / t["timed_loop_us_per_step"], rel=1e-9)
     # the dominant kernel runs four steps; the shorter ones are reported beside it
     assert rf["cell_updates_per_launch"] == 4 * 512 * 512
     for key in ("three_step_kernel", "two_step_kernel", "one_step_kernel"):
