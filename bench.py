@@ -321,6 +321,11 @@ def plain_section(args, dtype_name="f64"):
             "cell_update_rate_vs_one_step_roofline": value * bpc / HBM_PEAK,
             "valu_issue_us": valu_us,
             "valu_issue_frac": (valu_us / (kernK_s * 1e6)) if valu_us else None,
+            "kernel_us_per_step": kernK_s / K * 1e6,
+            "note": "temporal blocking: one read and one write of the fields per launch "
+                    "carry %d cell-updates per cell, so frac falls as K rises while the time "
+                    "per step falls; the launch is bound by FP64 arithmetic at two waves per "
+                    "SIMD (valu_issue_frac), not by HBM" % K,
             **others,
         },
     }
