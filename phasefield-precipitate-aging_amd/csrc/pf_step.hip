@@ -140,19 +140,46 @@ void step4_kernel(const T* __restrict__ iCr, const T* __restrict__ iNb,
                                               (LdsPtr)win_lds + w * PER_WAVE);
 }
 
+// Chunk height of the four-step kernel from a model of its launch on
+// MI355X: 256 CUs x 4 SIMDs hold 2048 waves at two waves per SIMD.  A wave's
+// time is proportional to its rows plus the 3 rows (K - 1) its levels
+// recompute per chunk; a SIMD running two waves delivers 1.3x one wave's
+// throughput, so a full round of resident waves costs 2/1.3 wave-times and a
+// last partial round of at most one wave per SIMD costs one.  The height
+// that minimises (rows + 3) x (rounds) is the measured best, or within 0.7%
+// of it, from 1024^2 to 8192^2 (e.g. 10 rows at 1024^2, 38 at 2048^2, 75 at
+// 4096^2, 119 at 8192^2).
+static int rows4_model(int strips, int Ny) {
+  constexpr long RESIDENT = 2048;
+  constexpr double PAIR = 2.0 / 1.3, HALO = 3.0;
+  int best = 1;
+  double best_cost = 1e300;
+  for (int rows = 1; rows <= min(Ny, 128); rows++) {
+    const long waves = (long)strips * ((Ny + rows - 1) / rows);
+    const long full = waves / RESIDENT, rem = waves % RESIDENT;
+    const double tail = rem == 0 ? 0.0 : (rem <= RESIDENT / 2 ? 1.0 : PAIR);
+    const double cost = (rows + HALO) * (PAIR * (double)full + tail);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = rows;
+    }
+  }
+  return best;
+}
+
 int default_rows_k(int k, int nx, int ny, int storage_bytes) {
   // The intermediate levels recompute (K-1)K rows per chunk in all, so chunks
   // are longer than the one-step kernel's while the grid still fills the chip.
   // Two steps: FP64 storage is near the HBM bound and prefers more waves in
   // flight (16 rows at 4096^2); FP32 storage is bound by arithmetic and
   // prefers less recomputation (32).  Three steps are bound by arithmetic in
-  // both and take 48 rows at 4096^2.  Four steps recompute more rows per
-  // chunk still and run at two waves per SIMD: 60 rows at 4096^2.
+  // both and take 48 rows at 4096^2.  Four steps: rows4_model.
   const int Nx = nx - 2, Ny = ny - 2;
   const int sw = WAVE - 2 * k;
   const int strips = (Nx + sw - 1) / sw;
-  int rows = k == 4 ? 60 : k == 3 ? 48 : (storage_bytes == 8 ? 16 : 32);
-  const long min_waves = k >= 3 ? 4096 : 8192;
+  if (k == 4) return rows4_model(strips, Ny);
+  int rows = k == 3 ? 48 : (storage_bytes == 8 ? 16 : 32);
+  const long min_waves = k == 3 ? 4096 : 8192;
   while (rows > 6 && (long)strips * ((Ny + rows - 1) / rows) < min_waves) rows >>= 1;
   return rows;
 }

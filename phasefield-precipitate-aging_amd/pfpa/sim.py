@@ -39,14 +39,22 @@ def iter_launches(t0, n, max_steps, nuc_every=0):
         t += k
 
 
+# FP32 storage: four steps per launch below this many interior cells, three
+# from it on (the four-step kernel is 5-15% faster at 1024^2-2048^2 and 3-4%
+# slower at 3072^2-8192^2, where the FP32 kernels are bound by arithmetic)
+FP32_FOUR_STEP_CELLS = 2560 * 2560
+
+
 def default_steps_per_launch(setup, dtype, requested=None):
     """Steps per launch of a Simulation: `requested` (1 to 4), or by default
-    four for FP64 storage and three for FP32 (the faster kernel of each on
-    the grids measured), reduced to what the grid and mask allow: the
-    multi-step kernels take symmetric five-point masks and an interior of at
-    least 3x3 (four steps: 4x4)."""
+    the faster kernel on the grids measured: four for FP64 storage, and for
+    FP32 four below FP32_FOUR_STEP_CELLS interior cells and three above;
+    reduced to what the grid and mask allow: the multi-step kernels take
+    symmetric five-point masks and an interior of at least 3x3 (four steps:
+    4x4)."""
     if requested is None:
-        requested = 4 if dtype == torch.float64 else 3
+        cells = (setup.nx - 2) * (setup.ny - 2)
+        requested = 4 if dtype == torch.float64 or cells < FP32_FOUR_STEP_CELLS else 3
     if requested not in (1, 2, 3, 4):
         raise ValueError("steps_per_launch must be 1, 2, 3 or 4")
     if not (M.symmetric_five_point(setup.mask) and setup.nx >= 5 and setup.ny >= 5):

@@ -79,16 +79,19 @@ def test_plan_launches_schedule_of_the_benchmark():
 
 
 def test_default_steps_per_launch():
-    """Four steps per launch for FP64 storage and three for FP32 by default;
-    an explicit request is kept where the grid and mask allow it, else cut to
-    three (interiors below 4x4) or one (below 3x3, or not a symmetric
-    five-point mask)."""
+    """Four steps per launch for FP64 storage by default, and for FP32 below
+    2560^2 interior cells (three from there); an explicit request is kept where
+    the grid and mask allow it, else cut to three (interiors below 4x4) or one
+    (below 3x3, or not a symmetric five-point mask)."""
     import torch
 
     from pfpa.sim import default_steps_per_launch as spl
 
     big, five, four = M.make_setup(64, 48, bc=1), M.make_setup(3, 9, bc=0), M.make_setup(4, 4, bc=1)
-    assert spl(big, torch.float64) == 4 and spl(big, torch.float32) == 3
+    assert spl(big, torch.float64) == 4 and spl(big, torch.float32) == 4
+    huge, edge = M.make_setup(8192, 8192, bc=1), M.make_setup(2560, 2560, bc=1)
+    assert spl(huge, torch.float64) == 4 and spl(huge, torch.float32) == 3
+    assert spl(edge, torch.float32) == 3 and spl(M.make_setup(2560, 2559, bc=1), torch.float32) == 4
     for k in (1, 2, 3, 4):
         assert spl(big, torch.float64, k) == k and spl(big, torch.float32, k) == k
     assert spl(four, torch.float64) == 4

@@ -294,3 +294,36 @@ def test_cpp_set_mask_matches_python_mirror(code):
             assert abs(sum(got)) > 1e-3 * max(abs(v) for v in got)
         else:
             assert abs(sum(got)) <= 1e-12 * max(abs(v) for v in got)
+
+
+def _rows4_model(Nx, Ny):
+    """The four-step kernel's chunk-height model, restated: 2048 resident
+    waves (256 CUs x 4 SIMDs x 2), a full round of them costs 2/1.3
+    wave-times, a last round of at most one wave per SIMD one wave-time, and
+    a wave's time is its rows plus the 3 its levels recompute."""
+    strips = -(-Nx // 56)
+    best, best_cost = 1, float("inf")
+    for rows in range(1, min(Ny, 128) + 1):
+        waves = strips * -(-Ny // rows)
+        full, rem = divmod(waves, 2048)
+        tail = 0.0 if rem == 0 else (1.0 if rem <= 1024 else 2 / 1.3)
+        cost = (rows + 3) * (2 / 1.3 * full + tail)
+        if cost < best_cost:
+            best, best_cost = rows, cost
+    return best
+
+
+def test_default_rows4_model():
+    """pf_default_rows4 follows the launch model, whose picks are the heights
+    measured fastest (or within 0.7% of them) at 1024^2..8192^2; the other
+    multi-step heuristics keep their documented values at 4096^2."""
+    L = pfpa.lib()
+    for N in (1024, 2048, 4096, 8192):
+        assert L.pf_default_rows4(N + 2, N + 2, 0) == {1024: 10, 2048: 38, 4096: 75,
+                                                        8192: 119}[N]
+    for Nx, Ny in ((4, 4), (5, 300), (300, 5), (256, 256), (1536, 1536), (3072, 3072),
+                   (4096, 512), (4096, 1024), (7, 5000), (10000, 64)):
+        assert L.pf_default_rows4(Nx + 2, Ny + 2, 0) == _rows4_model(Nx, Ny), (Nx, Ny)
+        assert L.pf_default_rows4(Nx + 2, Ny + 2, 1) == _rows4_model(Nx, Ny), (Nx, Ny)
+    assert L.pf_default_rows3(4098, 4098, 0) == 48
+    assert L.pf_default_rows2(4098, 4098, 0) == 16 and L.pf_default_rows2(4098, 4098, 1) == 32
