@@ -73,7 +73,7 @@ int pf_device_count(void);
 int pf_ctx_set_tile(pf_ctx* ctx, int rows, int xcd_remap, int depth);
 /* Composition offset of FP32 storage (no counterpart in the reference, whose
  * fields are FP64): on this context the conc_Cr / conc_Nb arrays of every
- * *_f32 entry point (step, two/three-step, nucleation, summarize, energy) hold
+ * *_f32 entry point (step, multi-step, nucleation, summarize, energy) hold
  * x - x_cr and x - x_nb.  With the offsets near the mean composition the FP32
  * values are ~10x smaller, so each step's small composition increment keeps
  * 3-4 more bits.  Default 0 (arrays hold x); FP64 entry points ignore it.

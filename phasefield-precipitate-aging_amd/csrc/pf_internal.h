@@ -119,8 +119,8 @@ hipError_t launch_step(int nx, int ny, int bc, const StepParams& P, const T* iCr
                        const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd, T* opl,
                        const NucParams* nuc, const Scratch& S, const TileConfig& tc,
                        hipStream_t st);
-// k = 2 or 3 steps per launch (folded symmetric five-point mask, no nucleation); grids need
-// Nx, Ny >= 3
+// k = 2, 3 or 4 steps per launch (folded symmetric five-point mask, no nucleation); grids
+// need Nx, Ny >= 3 (k = 4: >= 4)
 template <typename T>
 hipError_t launch_stepk(int k, int nx, int ny, int bc, const StepParams& P, const T* iCr,
                         const T* iNb, const T* ipd, const T* ipl, T* oCr, T* oNb, T* opd,

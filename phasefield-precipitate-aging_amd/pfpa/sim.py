@@ -1,4 +1,4 @@
-"""Native stepping loop over the C-ABI.  Steps run up to three per launch
+"""Native stepping loop over the C-ABI.  Steps run up to four per launch
 (the temporally blocked kernels) wherever no step of the group is a
 nucleation step; nucleation steps, the ones the reference's schedule selects
 (kernel_time % nuc_interval == 0, src/main.cpp:307-316), run one per launch

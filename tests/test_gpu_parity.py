@@ -158,13 +158,16 @@ def test_step_golden_fixture_periodic(gpu, step_golden):
             full = np.abs(got[k] - F[step][k])
             assert full.max() <= TOL * max(np.abs(F[step][k]).max(), 1.0), (step, k)
     sim.close()
-    # and all four steps in one three-step plus one one-step launch
-    sim = Simulation(s, [F[0][k] for k in range(4)])
-    sim.step(4)
-    got = sim.fields()
-    for k in range(4):
-        assert mixed_err(got[k], F[4][k]) <= TOL, k
-    sim.close()
+    # and all four steps in one four-step launch (the FP64 default), and in
+    # one three-step plus one one-step launch
+    for spl in (4, 3):
+        sim = Simulation(s, [F[0][k] for k in range(4)], steps_per_launch=spl)
+        assert sim.steps_per_launch == spl
+        sim.step(4)
+        got = sim.fields()
+        for k in range(4):
+            assert mixed_err(got[k], F[4][k]) <= TOL, (spl, k)
+        sim.close()
 
 
 @pytest.mark.parametrize("bc", [1, 0])
@@ -412,13 +415,13 @@ def test_mass_conservation_4096(gpu):
 def test_parity_4096_benchmark_workload(gpu, oracle):
     """The benchmark's own workload (4096^2 periodic, enriched band, nucleation
     on: step 0 is a nucleation step, whose P rounds to 0) for 8 steps, which the
-    stepping loop runs as one fused nucleation launch, two three-step launches
-    and one one-step launch: every cell against the oracle."""
+    stepping loop runs as one fused nucleation launch, one four-step launch and
+    one three-step launch: every cell against the oracle."""
     s = M.make_setup(4096, 4096, bc=1)
     f0 = M.init_enriched_band(4096, 4096, 1, seed=42)
     sim = Simulation(s, f0, nucleation=True)
-    assert list(iter_launches(0, 8, 3, sim.nuc_every)) == [(1, True), (3, False), (3, False),
-                                                          (1, False)]
+    assert sim.steps_per_launch == 4
+    assert list(iter_launches(0, 8, 4, sim.nuc_every)) == [(1, True), (4, False), (3, False)]
     sim.step(8)
     got = sim.fields()
     sim.close()
@@ -431,9 +434,10 @@ def test_largest_grid_fp32(gpu):
     """The largest square grid the 32-bit cell index allows (46340^2 with the
     ghost ring, 2.1e9 cells; FP32 storage, 69 GB of fields): a perturbation
     at the last interior cell, whose periodic neighbours wrap to the start of
-    the arrays, evolves over a three-step and a one-step launch exactly as the
-    same perturbation at the corner of a 64^2 grid."""
-    def run(n, pos):
+    the arrays, evolves over a three-step and a one-step launch, and over one
+    four-step launch, exactly as the same perturbation at the corner of a 64^2
+    grid."""
+    def run(n, pos, four):
         s = M.make_setup(n - 2, n - 2, bc=1)
         P = PfParams()
         P.dt = s.dt
@@ -456,10 +460,15 @@ def test_largest_grid_fp32(gpu):
         b = [torch.empty_like(t) for t in a]
         ctx = ctypes.c_void_p()
         check(lib().pf_ctx_create(ctypes.byref(ctx)), "ctx")
-        check(lib().pf_step3_f32(ctx, n, n, 1, ctypes.byref(P), *map(_p, a), *map(_p, b),
-                                 None), "step3")
-        check(lib().pf_step_f32(ctx, n, n, 1, ctypes.byref(P), *map(_p, b), *map(_p, a),
-                                None, None), "step")
+        if four:
+            check(lib().pf_step4_f32(ctx, n, n, 1, ctypes.byref(P), *map(_p, a), *map(_p, b),
+                                     None), "step4")
+            a, b = b, a
+        else:
+            check(lib().pf_step3_f32(ctx, n, n, 1, ctypes.byref(P), *map(_p, a), *map(_p, b),
+                                     None), "step3")
+            check(lib().pf_step_f32(ctx, n, n, 1, ctypes.byref(P), *map(_p, b), *map(_p, a),
+                                    None, None), "step")
         torch.cuda.synchronize()
         lib().pf_ctx_destroy(ctx)
         # the 11 x 11 interior neighbourhood of the last interior cell, wrapped
@@ -474,18 +483,23 @@ def test_largest_grid_fp32(gpu):
 
     big = 46340
     assert big * big < 2 ** 31 <= (big + 1) * (big + 1)
-    small_out, small_far = run(64, (62, 62))
-    big_out, big_far = run(big, (big - 2, big - 2))
+    small_out, small_far = run(64, (62, 62), False)
+    small4_out, small4_far = run(64, (62, 62), True)
     for k in range(4):
-        assert np.array_equal(big_out[k], small_out[k]), k
-        assert big_far[k] == small_far[k], k
+        assert np.array_equal(small4_out[k], small_out[k]), k
+    for four in (False, True):
+        big_out, big_far = run(big, (big - 2, big - 2), four)
+        for k in range(4):
+            assert np.array_equal(big_out[k], small_out[k]), (four, k)
+            assert big_far[k] == small_far[k], (four, k)
     assert np.abs(small_out[2] - small_out[2][0, 0]).max() > 0   # the bump has spread
 
 
 def test_precision_form_8192(gpu):
     """The precision workload's size (8192^2, the plain workload's
     initialisation, periodic): FP32 storage and FP64 from the same fields for
-    30 steps (ten three-step launches).  FP64 conserves each composition sum to
+    30 steps (FP32: ten three-step launches; FP64: seven four-step launches and
+    a two-step one).  FP64 conserves each composition sum to
     rounding; FP32 storage stays within a few FP32 ulps of x per step of FP64
     and conserves the sums to FP32 rounding."""
     N = 8192
