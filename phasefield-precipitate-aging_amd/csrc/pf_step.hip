@@ -118,7 +118,7 @@ __global__ __launch_bounds__(WAVE * WPB) void stepk_kernel(
                           b / strips, rows, P);
 }
 
-// Four steps: the rows behind of the four levels' windows live in LDS (32 KiB
+// Four steps: the rows behind of the four levels' windows live in LDS (16 KiB
 // per wave, 64 KiB per block of WPB = 4 waves), which caps the kernel at two
 // blocks, so two waves per SIMD, per CU; the register budget is set to match.
 template <typename T, int BC, int DEPTH>
