@@ -141,7 +141,7 @@ def cpu_baseline(args):
     may use (OMP_PROC_BIND=close) and on one core."""
     import subprocess
     cpu = host_cpu()
-    # every physical core, within the 16-CPU share a one-GPU box gives a job
+    # every physical core, within the 16-CPU share a one-GPU host gives a job
     threads = min(16, cpu["physical_cores"] or cpu["logical_cpus"])
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
                OMP_PLACES="cores")

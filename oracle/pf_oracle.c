@@ -38,7 +38,7 @@
  *                                     exactly from the known-answer words
  *   diffusion matrix, mobility L      golden D_* at (0.30, 0.02); L against the
  *                                     closed form on those values, 1/w^2 scaling
- *                                     and the survey's 4.0015e-8
+ *                                     and the hand-evaluated 4.0015e-8
  *   field summary                     uniform interiors (exact p(0), p(1/2), p(1)),
  *                                     NaN / 1e300 ghost rings, dyadic fields with
  *                                     hand-computed means

@@ -433,7 +433,7 @@ def test_diffusion_matrix_and_mobility_pinned(oracle, thermo_golden):
     assert DCr[1] == pytest.approx(G["D_CrNb"][0], rel=1e-14)
     assert DNb[0] == pytest.approx(G["D_NbCr"][0], rel=1e-14)
     assert DNb[1] == pytest.approx(G["D_NbNb"][0], rel=1e-14)
-    # the survey's evaluation of the same expression: L = 4.0015e-8 m^2/(N s)
+    # the same expression evaluated by hand at (0.30, 0.02): L = 4.0015e-8 m^2/(N s)
     assert L[0] == pytest.approx(4.0015e-8, rel=1e-4)
     assert L[1] == L[0]
     # the norm is over the diagonal (D_CrCr, D_NbNb), not a row or column: the
