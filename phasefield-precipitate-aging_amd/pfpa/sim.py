@@ -176,17 +176,24 @@ class Simulation:
             raise ValueError("tune_tile3 needs three steps per launch")
         return self.tune_tile(rows_set, launches)
 
-    def tune_tile(self, rows_set=(6, 8, 12, 16, 24, 32, 48, 60, 64, 72), launches=20):
+    def tune_tile(self, rows_set=None, launches=20):
         """Time the simulation's multi-step kernel (steps_per_launch 2, 3 or
         4) at each chunk height in `rows_set` on scratch copies of the
         current fields (the simulation does not advance) and keep the
-        fastest.  The default height is a heuristic within a few per cent of
-        the best on the grids measured; this measures the grid at hand.
-        Results do not depend on the tile.  Returns (rows, {rows: us per
-        launch})."""
+        fastest.  The default heights (a launch model for four steps, a
+        heuristic for two and three) are within a few per cent of the best on
+        the grids measured; this measures the grid at hand.  rows_set None:
+        the kernel's default height and 6, 8, 12, 16, 24, 32, 48, 60, 64, 72,
+        96 (those up to the interior height).  Results do not depend on the
+        tile.  Returns (rows, {rows: us per launch})."""
         k = self.steps_per_launch
         if k < 2:
             raise ValueError("tune_tile needs two or more steps per launch")
+        if rows_set is None:
+            storage = 0 if self.dtype == torch.float64 else 1
+            default = getattr(lib(), "pf_default_rows%d" % k)(self.nx, self.ny, storage)
+            rows_set = sorted({default} | {r for r in (6, 8, 12, 16, 24, 32, 48, 60, 64, 72, 96)
+                                          if r <= self.ny - 2})
         fn = self._stepk[k]
         set_tile = {2: self.set_tile2, 3: self.set_tile3, 4: self.set_tile4}[k]
         st = ctypes.c_void_p(self.stream.cuda_stream)

@@ -251,6 +251,10 @@ def test_tune_tile_leaves_fields_and_results(gpu, spl):
     if spl == 4:
         with pytest.raises(ValueError):
             tuned.tune_tile3()
+        # the default candidates include the kernel's own default height
+        rows, times = tuned.tune_tile(launches=2)
+        default = lib().pf_default_rows4(s.nx, s.ny, 0)
+        assert default in times and rows in times and max(times) <= 200
     tuned.close()
     plain.close()
 
