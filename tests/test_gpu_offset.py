@@ -137,15 +137,18 @@ def test_offset_fused_nucleation_matches_separate_pass(gpu, oracle):
     lib().pf_ctx_destroy(ctx)
 
 
+@pytest.mark.parametrize("spl", [3, 4])
 @pytest.mark.parametrize("bc", [0, 1])
-def test_offset_multistep_equals_one_step(gpu, bc):
-    """Three steps per launch on offset storage == one launch per step, with
-    the nucleation schedule on (fused nucleation launches in between)."""
+def test_offset_multistep_equals_one_step(gpu, bc, spl):
+    """Three and four steps per launch on offset storage == one launch per
+    step, with the nucleation schedule on (fused nucleation launches in
+    between)."""
     Nx, Ny = 150, 97
     s = M.make_setup(Nx, Ny, bc=bc)
     f0 = M.init_matrix_with_particles(Nx, Ny, s.dx, bc, seed=3, radius=2.5e-9)
     a = Simulation(s, f0, dtype=torch.float32, nucleation=True, nuc_every=7,
-                   steps_per_launch=3, storage_offset="mean")
+                   steps_per_launch=spl, storage_offset="mean")
+    assert a.steps_per_launch == spl
     b = Simulation(s, f0, dtype=torch.float32, nucleation=True, nuc_every=7,
                    steps_per_launch=1, storage_offset="mean")
     assert a.xoff == b.xoff and 0.25 < a.xoff[0] < 0.35 and 0.02 < a.xoff[1] < 0.1
